@@ -1,0 +1,219 @@
+"""NumPy/SciPy restatement of serinv's sequential SPD BT/BTA routines.
+
+TEST INFRASTRUCTURE ONLY (see oracle/__init__.py).
+
+Every function works in place on C-contiguous float64 block arrays with the
+reference layout:
+    diagonal blocks          (n, b, b)
+    lower-diagonal blocks    (n-1, b, b)
+    lower arrow blocks       (n, a, b)
+    arrow tip block          (a, a)
+    right-hand side B        (n*b + a, r)   (n*b, r) for the BT solve
+and follows the natural-ordering, non-streaming reference routine step for
+step.  The Cholesky is scipy.linalg.cholesky(lower=True, overwrite_a=False,
+check_finite=False), exactly the function the reference selects for NumPy
+arrays (src/serinv/__init__.py:137-155, line 151); it zero-fills the upper
+triangle.  Triangular solves are scipy.linalg.solve_triangular with the same
+flags the reference passes.
+
+Parity status: every function here is pinned by a dense-linear-algebra check
+in tests/test_oracle_pinning.py (see the module docstring of oracle/).
+"""
+
+from functools import partial
+
+import numpy as np
+import scipy.linalg as sla
+
+# src/serinv/__init__.py:151
+_cholesky = partial(sla.cholesky, lower=True, overwrite_a=False, check_finite=False)
+
+
+def _check_kwargs(kwargs, allowed):
+    # src/serinv/algs/pobtaf.py:50-53, pobtasi.py:45-48, pobtf.py:44-47,
+    # pobtsi.py:39-42: unknown keyword arguments raise TypeError.
+    unexpected = set(kwargs) - set(allowed)
+    if unexpected:
+        raise TypeError(f"Unexpected keyword arguments: {unexpected}")
+
+
+def _reject_unsupported(kwargs):
+    # The oracle restates only the natural, in-memory branch
+    # (src/serinv/algs/pobtaf.py:87-94 and the matching branches elsewhere).
+    if kwargs.get("buffer", None) is not None:
+        raise NotImplementedError("permuted (buffer=...) variants are not restated")
+    if kwargs.get("device_streaming", False):
+        raise NotImplementedError("device streaming is not restated")
+
+
+# --------------------------------------------------------------------------
+# BTA: pobtaf / pobtasi / pobtas
+# --------------------------------------------------------------------------
+def pobtaf(D, L, R, T, **kwargs):
+    """Block Cholesky of an SPD BTA matrix, in place.
+
+    Follows src/serinv/algs/pobtaf.py:11-94 (dispatch) and :97-183 (_pobtaf).
+    """
+    _check_kwargs(kwargs, {"device_streaming", "buffer", "factorize_last_block"})
+    _reject_unsupported(kwargs)
+    factorize_last_block = kwargs.get("factorize_last_block", True)
+    n = D.shape[0]
+    for i in range(n - 1):  # pobtaf.py:115
+        D[i] = _cholesky(D[i])  # :117
+        # L_{i+1,i} = A_{i+1,i} L_ii^{-T}  (:120-128)
+        L[i] = sla.solve_triangular(D[i], L[i].conj().T, lower=True).conj().T
+        # L_{t,i} = A_{t,i} L_ii^{-T}  (:131-139)
+        R[i] = sla.solve_triangular(D[i], R[i].conj().T, lower=True).conj().T
+        # Schur updates (:143-159), in the reference's order
+        D[i + 1] = D[i + 1] - L[i] @ L[i].conj().T
+        R[i + 1] = R[i + 1] - R[i] @ L[i].conj().T
+        T[:, :] = T - R[i] @ R[i].conj().T
+    if factorize_last_block:  # :161-183
+        D[-1] = _cholesky(D[-1])
+        R[-1] = sla.solve_triangular(D[-1], R[-1].conj().T, lower=True).conj().T
+        T[:, :] = T - R[-1] @ R[-1].conj().T
+        T[:, :] = _cholesky(T)
+
+
+def pobtasi(D, L, R, T, **kwargs):
+    """Selected inversion of an SPD BTA matrix from its block Cholesky factor.
+
+    Follows src/serinv/algs/pobtasi.py:10-89 (dispatch) and :92-172 (_pobtasi).
+    On exit D holds X_ii (full symmetric), L holds X_{i+1,i}, R holds X_{t,i}
+    and T holds X_T.
+    """
+    _check_kwargs(kwargs, {"device_streaming", "buffer", "invert_last_block"})
+    _reject_unsupported(kwargs)
+    invert_last_block = kwargs.get("invert_last_block", True)
+    n, b = D.shape[0], D.shape[1]
+    eye_b = np.eye(b, dtype=D.dtype)  # pobtasi.py:112
+    if invert_last_block:  # :114-139
+        LT_inv = sla.solve_triangular(T, np.eye(T.shape[0], dtype=T.dtype), lower=True)
+        T[:, :] = LT_inv.conj().T @ LT_inv  # :119
+        Lt_last = R[-1].copy()  # :122
+        Linv = sla.solve_triangular(D[-1], eye_b, lower=True)  # :124-128
+        R[-1] = -T @ Lt_last @ Linv  # :131-133
+        D[-1] = (Linv.conj().T - R[-1].conj().T @ Lt_last) @ Linv  # :136-139
+    for i in range(n - 2, -1, -1):  # :141
+        Li = L[i].copy()  # :142
+        Lti = R[i].copy()  # :143
+        Linv = sla.solve_triangular(D[i], eye_b, lower=True)  # :145-149
+        L[i] = (-D[i + 1] @ Li - R[i + 1].conj().T @ Lti) @ Linv  # :153-156
+        R[i] = (-R[i + 1] @ Li - T @ Lti) @ Linv  # :160-163
+        D[i] = (Linv.conj().T - L[i].conj().T @ Li - R[i].conj().T @ Lti) @ Linv  # :167-172
+
+
+def pobtas(D, L, R, T, B, trans="N", **kwargs):
+    """Forward (trans="N") or backward (trans="T"/"C") BTA triangular solve.
+
+    Follows src/serinv/algs/pobtas.py:10-62 (dispatch) and :65-161 (_pobtas).
+    The reference does not validate kwargs for pobtas.
+    """
+    partial_ = kwargs.get("partial", False)
+    if kwargs.get("buffer", None) is not None:
+        raise NotImplementedError("permuted (buffer=...) variant is not restated")
+    if kwargs.get("device_streaming", False):
+        # pobtas.py:49-52
+        raise NotImplementedError("Streaming is not implemented for the natural arrowhead.")
+    n, b, a = D.shape[0], D.shape[1], R.shape[1]
+    blk = lambda i: slice(i * b, (i + 1) * b)  # noqa: E731
+    tip = slice(n * b, n * b + a)
+    if trans == "N":  # :80-126
+        for i in range(n - 1):
+            B[blk(i)] = sla.solve_triangular(D[i], B[blk(i)], lower=True)
+            B[blk(i + 1)] -= L[i] @ B[blk(i)]
+            B[tip] -= R[i] @ B[blk(i)]
+        if not partial_:
+            B[blk(n - 1)] = sla.solve_triangular(D[n - 1], B[blk(n - 1)], lower=True)
+            B[tip] -= R[-1] @ B[blk(n - 1)]
+            B[tip] = sla.solve_triangular(T, B[tip], lower=True)
+    elif trans == "T" or trans == "C":  # :127-159
+        if not partial_:
+            B[tip] = sla.solve_triangular(T, B[tip], lower=True, trans="C")
+            B[blk(n - 1)] = sla.solve_triangular(
+                D[-1], B[blk(n - 1)] - R[-1].conj().T @ B[tip], lower=True, trans="C"
+            )
+        for i in range(n - 2, -1, -1):
+            B[blk(i)] = sla.solve_triangular(
+                D[i],
+                B[blk(i)] - L[i].conj().T @ B[blk(i + 1)] - R[i].conj().T @ B[tip],
+                lower=True,
+                trans="C",
+            )
+    else:
+        raise ValueError(f"Invalid transpose argument: {trans}.")  # :160-161
+
+
+# --------------------------------------------------------------------------
+# BT (arrowless): pobtf / pobtsi / pobts
+# --------------------------------------------------------------------------
+def pobtf(D, L, **kwargs):
+    """Block Cholesky of an SPD BT matrix (src/serinv/algs/pobtf.py:11-122)."""
+    _check_kwargs(kwargs, {"device_streaming", "buffer", "factorize_last_block"})
+    _reject_unsupported(kwargs)
+    factorize_last_block = kwargs.get("factorize_last_block", True)
+    n = D.shape[0]
+    for i in range(n - 1):  # pobtf.py:97
+        D[i] = _cholesky(D[i])  # :99
+        L[i] = sla.solve_triangular(D[i], L[i].conj().T, lower=True).conj().T  # :102-110
+        D[i + 1] = D[i + 1] - L[i] @ L[i].conj().T  # :114-118
+    if factorize_last_block:
+        D[-1] = _cholesky(D[-1])  # :122
+
+
+def pobtsi(D, L, **kwargs):
+    """Selected inversion of an SPD BT matrix (src/serinv/algs/pobtsi.py:10-125)."""
+    _check_kwargs(kwargs, {"device_streaming", "buffer", "invert_last_block"})
+    _reject_unsupported(kwargs)
+    invert_last_block = kwargs.get("invert_last_block", True)
+    n, b = D.shape[0], D.shape[1]
+    eye_b = np.eye(b, dtype=D.dtype)  # pobtsi.py:92
+    if invert_last_block:  # :94-102
+        Linv = sla.solve_triangular(D[-1], eye_b, lower=True)
+        D[-1] = Linv.conj().T @ Linv
+    for i in range(n - 2, -1, -1):  # :104
+        Li = L[i].copy()  # :105
+        Linv = sla.solve_triangular(D[i], eye_b, lower=True)  # :107-111
+        L[i] = (-D[i + 1] @ Li) @ Linv  # :115-117
+        D[i] = (Linv.conj().T - L[i].conj().T @ Li) @ Linv  # :121-125
+
+
+def pobts(D, L, B, trans="N", **kwargs):
+    """Forward/backward BT triangular solve (src/serinv/algs/pobts.py:10-118)."""
+    partial_ = kwargs.get("partial", False)
+    if kwargs.get("buffer", None) is not None:
+        raise NotImplementedError("permuted (buffer=...) variant is not restated")
+    if kwargs.get("device_streaming", False):
+        raise NotImplementedError("Streaming is not implemented for the natural arrowhead.")
+    n, b = D.shape[0], D.shape[1]
+    blk = lambda i: slice(i * b, (i + 1) * b)  # noqa: E731
+    if trans == "N":  # pobts.py:69-95
+        for i in range(n - 1):
+            B[blk(i)] = sla.solve_triangular(D[i], B[blk(i)], lower=True)
+            B[blk(i + 1)] -= L[i] @ B[blk(i)]
+        if not partial_:
+            B[blk(n - 1)] = sla.solve_triangular(D[n - 1], B[blk(n - 1)], lower=True)
+    elif trans == "T" or trans == "C":  # :96-116
+        if not partial_:
+            B[blk(n - 1)] = sla.solve_triangular(D[-1], B[blk(n - 1)], lower=True, trans="C")
+        for i in range(n - 2, -1, -1):
+            B[blk(i)] = sla.solve_triangular(
+                D[i], B[blk(i)] - L[i].conj().T @ B[blk(i + 1)], lower=True, trans="C"
+            )
+    else:
+        raise ValueError(f"Invalid transpose argument: {trans}.")  # :117-118
+
+
+# --------------------------------------------------------------------------
+# Build-side extension (no reference symbol): log-determinant from the factor
+# --------------------------------------------------------------------------
+def logdet_from_factor(D, T=None):
+    """log det A = 2 * sum(log diag L_ii) + 2 * sum(log diag L_T).
+
+    D holds the pobtaf/pobtf output (lower Cholesky blocks); T the tip factor
+    (None or empty for BT).  Pinned by numpy.linalg.slogdet.
+    """
+    s = 2.0 * np.sum(np.log(np.diagonal(D, axis1=1, axis2=2)))
+    if T is not None and T.size:
+        s += 2.0 * np.sum(np.log(np.diag(T)))
+    return float(s)
