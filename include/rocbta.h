@@ -1,0 +1,85 @@
+/*
+ * rocbta: MI355X (gfx950) SPD block-tridiagonal-with-arrowhead (BTA) kernels.
+ *
+ * C ABI for the drop-in replacement of serinv's sequential SPD BTA/BT path.
+ * All matrix pointers are DEVICE pointers to C-contiguous (row-major) float64
+ * arrays in serinv's block layout:
+ *   D  diagonal blocks        (n, b, b)
+ *   L  lower-diagonal blocks  (n-1, b, b)
+ *   R  lower arrow blocks     (n, a, b)
+ *   T  arrow tip              (a, a)
+ *   B  right-hand sides       (n*b + a, r), row stride ldb (>= r)
+ * a == 0 selects the arrowless BT routines (pobtf / pobtsi / pobts); R and T
+ * may then be NULL.  `stream` is a hipStream_t; every call is stream-ordered
+ * and returns without synchronizing.  Return value: ROCBTA_OK or a negative
+ * ROCBTA_ERR_* code (argument errors are detected on the host before any
+ * launch).  Numerical failure of a Cholesky pivot is reported through the
+ * device int `*info` (0 = success, else 1 + global row index of the first
+ * non-positive pivot, as LAPACK potrf).
+ *
+ * Workspace: the caller provides device memory of at least
+ * rocbta_workspace_bytes(op, n, b, a, r) bytes, 256-byte aligned.
+ */
+#ifndef ROCBTA_H
+#define ROCBTA_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define ROCBTA_OK 0
+#define ROCBTA_ERR_ARG (-1)
+#define ROCBTA_ERR_HIP (-2)
+#define ROCBTA_ERR_WORKSPACE (-3)
+
+#define ROCBTA_OP_POBTAF 0
+#define ROCBTA_OP_POBTASI 1
+#define ROCBTA_OP_POBTAS 2
+
+/* Library version string. */
+const char* rocbta_version(void);
+
+/* Bytes of device workspace an operation needs for the given shape. */
+size_t rocbta_workspace_bytes(int op, int64_t n, int64_t b, int64_t a, int64_t r);
+
+/* Block Cholesky factorization, in place.
+ * Replaces serinv.algs.pobtaf / _pobtaf (src/serinv/algs/pobtaf.py:11-94,
+ * 97-183) and, with a == 0, serinv.algs.pobtf / _pobtf (pobtf.py:11-122).
+ * factorize_last_block = 0 leaves the Schur complements in D[n-1], R[n-1], T
+ * (pobtaf.py:161). */
+int rocbta_dpobtaf(int64_t n, int64_t b, int64_t a, double* D, double* L, double* R, double* T,
+                   int factorize_last_block, int* info, void* workspace, size_t workspace_bytes,
+                   void* stream);
+
+/* Selected inversion from the block Cholesky factor, in place.
+ * Replaces serinv.algs.pobtasi / _pobtasi (src/serinv/algs/pobtasi.py:10-89,
+ * 92-172) and, with a == 0, serinv.algs.pobtsi / _pobtsi (pobtsi.py:10-125).
+ * invert_last_block = 0 takes D[n-1], R[n-1], T as already-inverted blocks
+ * (pobtasi.py:114). */
+int rocbta_dpobtasi(int64_t n, int64_t b, int64_t a, double* D, double* L, double* R, double* T,
+                    int invert_last_block, void* workspace, size_t workspace_bytes, void* stream);
+
+/* Triangular solve with the block Cholesky factor, in place on B.
+ * trans 'N': B <- L^{-1} B ; 'T' or 'C': B <- L^{-T} B.
+ * Replaces serinv.algs.pobtas / _pobtas (src/serinv/algs/pobtas.py:10-62,
+ * 65-161) and, with a == 0, serinv.algs.pobts / _pobts (pobts.py:10-118).
+ * partial = 1 skips the last-block and tip solves (pobtas.py:99,129). */
+int rocbta_dpobtas(int64_t n, int64_t b, int64_t a, const double* D, const double* L,
+                   const double* R, const double* T, double* B, int64_t r, int64_t ldb,
+                   char trans, int partial, void* workspace, size_t workspace_bytes,
+                   void* stream);
+
+/* log det A = 2 sum log diag(L_ii) + 2 sum log diag(L_T) from the factor
+ * written by rocbta_dpobtaf; result stored to the device double *out.
+ * No counterpart in the reference (extension for likelihood evaluation). */
+int rocbta_dlogdet(int64_t n, int64_t b, int64_t a, const double* D, const double* T,
+                   double* out, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* ROCBTA_H */

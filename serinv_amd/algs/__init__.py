@@ -1,0 +1,16 @@
+"""serinv-compatible sequential SPD BTA / BT routines on MI355X.
+
+Drop-in counterparts of serinv.algs (src/serinv/algs/__init__.py:5-31) for
+the SPD path: same names, argument order, in-place block layout, keyword
+arguments and error types, on float64 torch tensors on a ROCm device.
+"""
+
+from ._common import config
+from .pobtaf import pobtaf
+from .pobtas import pobtas
+from .pobtasi import pobtasi
+from .pobtf import pobtf
+from .pobts import pobts
+from .pobtsi import pobtsi
+
+__all__ = ["pobtaf", "pobtasi", "pobtas", "pobtf", "pobtsi", "pobts", "config"]

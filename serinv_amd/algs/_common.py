@@ -1,0 +1,98 @@
+"""Shared argument handling for the drop-in routines.
+
+Arrays are torch tensors on a ROCm device (float64, C-contiguous) in serinv's
+block layout; the routines overwrite them in place and return None, like the
+reference (src/serinv/algs/*.py).  There is no host path and no fallback: a
+tensor that is not on the GPU, or a missing librocbta.so, raises.
+"""
+
+import numpy as np
+import torch
+
+from .. import _lib
+
+# Runtime switches (module level, so that the reference's call signatures stay
+# unchanged).  check_info: after a factorization, read the device pivot status
+# back and raise numpy.linalg.LinAlgError on a non-positive pivot, as the
+# reference's NumPy path does (scipy.linalg.cholesky raises).  Turning it off
+# keeps the call fully asynchronous, as the reference's CuPy path is.
+config = {"check_info": True}
+
+
+def check_kwargs(kwargs, expected):
+    # src/serinv/algs/pobtaf.py:50-53 (same pattern in pobtasi/pobtf/pobtsi)
+    unexpected = set(kwargs) - set(expected)
+    if unexpected:
+        raise TypeError(f"Unexpected keyword arguments: {unexpected}")
+
+
+def reject_unsupported(kwargs, what):
+    if kwargs.get("buffer", None) is not None:
+        raise NotImplementedError(f"{what}: the permuted (buffer=...) variant is not implemented")
+    if kwargs.get("device_streaming", False):
+        raise NotImplementedError(f"{what}: host-device streaming is not implemented")
+
+
+def _tensor(x, name):
+    if not isinstance(x, torch.Tensor):
+        raise TypeError(f"{name} must be a torch.Tensor on a ROCm device, got {type(x).__name__}")
+    if x.device.type != "cuda":
+        raise TypeError(f"{name} must live on the GPU (got device {x.device})")
+    if x.dtype != torch.float64:
+        raise TypeError(f"{name} must be float64 (got {x.dtype})")
+    if not x.is_contiguous():
+        raise ValueError(f"{name} must be C-contiguous")
+    return x
+
+
+def bta_shapes(D, L, R, T):
+    """Validate the BTA block arrays; return (n, b, a, device)."""
+    _tensor(D, "diagonal blocks")
+    if D.dim() != 3 or D.shape[1] != D.shape[2]:
+        raise ValueError(f"diagonal blocks must have shape (n, b, b), got {tuple(D.shape)}")
+    n, b = D.shape[0], D.shape[1]
+    if n < 1 or b < 1:
+        raise ValueError("need at least one diagonal block of size >= 1")
+    _tensor(L, "lower diagonal blocks")
+    if tuple(L.shape) != (n - 1, b, b):
+        raise ValueError(f"lower diagonal blocks must have shape {(n - 1, b, b)}, got {tuple(L.shape)}")
+    if R is None:
+        a = 0
+    else:
+        _tensor(R, "lower arrow blocks")
+        if R.dim() != 3 or R.shape[0] != n or R.shape[2] != b:
+            raise ValueError(f"lower arrow blocks must have shape (n, a, b) = ({n}, a, {b}), got {tuple(R.shape)}")
+        a = R.shape[1]
+        _tensor(T, "arrow tip block")
+        if tuple(T.shape) != (a, a):
+            raise ValueError(f"arrow tip block must have shape {(a, a)}, got {tuple(T.shape)}")
+    devs = {t.device for t in (D, L, R, T) if t is not None}
+    if len(devs) != 1:
+        raise ValueError(f"all blocks must be on the same device, got {devs}")
+    return n, b, a, D.device
+
+
+def ptr(x):
+    return None if x is None or x.numel() == 0 else x.data_ptr()
+
+
+def stream_of(device):
+    return torch.cuda.current_stream(device).cuda_stream
+
+
+def workspace(op, n, b, a, r, device):
+    nbytes = _lib.load().rocbta_workspace_bytes(op, n, b, a, r)
+    if nbytes == 0:
+        raise ValueError(f"unsupported shape n={n} b={b} a={a}")
+    return torch.empty(nbytes, dtype=torch.uint8, device=device), nbytes
+
+
+def raise_on_info(info, what):
+    if not config["check_info"]:
+        return
+    v = int(info.item())
+    if v != 0:
+        # numpy/scipy Cholesky raise numpy.linalg.LinAlgError (the reference's
+        # host path, src/serinv/__init__.py:151)
+        raise np.linalg.LinAlgError(
+            f"{what}: the leading minor of order {v} is not positive definite")

@@ -1,0 +1,45 @@
+"""Block Cholesky factorization of an SPD BTA matrix (drop-in for serinv.algs.pobtaf).
+
+Reference: src/serinv/algs/pobtaf.py:11-94 (signature, kwargs, dispatch) and
+:97-183 (natural-ordering algorithm).  The factorization runs on the GPU as
+hand-written gfx950 HIP kernels (librocbta.so, rocbta_dpobtaf).
+"""
+
+import torch
+
+from .. import _lib
+from ._common import (bta_shapes, check_kwargs, config, ptr, raise_on_info, reject_unsupported,
+                      stream_of, workspace)
+
+
+def pobtaf(A_diagonal_blocks, A_lower_diagonal_blocks, A_lower_arrow_blocks, A_arrow_tip_block,
+           **kwargs):
+    """Cholesky factorization of a block tridiagonal with arrowhead matrix, in place.
+
+    Parameters follow serinv: diagonal blocks (n, b, b), lower diagonal blocks
+    (n-1, b, b), lower arrow blocks (n, a, b), arrow tip (a, a); float64 torch
+    tensors on a ROCm device.  On exit they hold L_ii (upper triangle zero),
+    L_{i+1,i}, L_{t,i} and L_T.
+
+    Keyword arguments: factorize_last_block (default True); device_streaming
+    and buffer are accepted names but only their defaults are supported.
+    A non-positive pivot raises numpy.linalg.LinAlgError (when
+    serinv_amd.algs.config["check_info"] is True).
+    """
+    check_kwargs(kwargs, {"device_streaming", "buffer", "factorize_last_block"})
+    reject_unsupported(kwargs, "pobtaf")
+    factorize_last_block = bool(kwargs.get("factorize_last_block", True))
+    D, L, R, T = A_diagonal_blocks, A_lower_diagonal_blocks, A_lower_arrow_blocks, A_arrow_tip_block
+    n, b, a, dev = bta_shapes(D, L, R, T)
+    _factor(D, L, R, T, n, b, a, dev, factorize_last_block, "pobtaf")
+
+
+def _factor(D, L, R, T, n, b, a, dev, factorize_last_block, what):
+    lib = _lib.load()
+    ws, nbytes = workspace(_lib.OP_POBTAF, n, b, a, 0, dev)
+    info = torch.empty(1, dtype=torch.int32, device=dev)
+    rc = lib.rocbta_dpobtaf(n, b, a, ptr(D), ptr(L), ptr(R), ptr(T), int(factorize_last_block),
+                            info.data_ptr(), ws.data_ptr(), nbytes, stream_of(dev))
+    _lib.check(rc, what)
+    if config["check_info"]:
+        raise_on_info(info, what)

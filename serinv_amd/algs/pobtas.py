@@ -1,0 +1,53 @@
+"""Triangular solve with the BTA block Cholesky factor (drop-in for serinv.algs.pobtas).
+
+Reference: src/serinv/algs/pobtas.py:10-62 (signature, kwargs, dispatch) and
+:65-161 (algorithm); runs as rocbta_dpobtas.  Like the reference, pobtas does
+not validate its keyword arguments.
+"""
+
+from .. import _lib
+from ._common import _tensor, bta_shapes, ptr, stream_of, workspace
+
+
+def pobtas(L_diagonal_blocks, L_lower_diagonal_blocks, L_lower_arrow_blocks, L_arrow_tip_block, B,
+           trans="N", **kwargs):
+    """Solve with the block Cholesky factor, in place on B (shape (n*b + a, r) or (n*b + a,)).
+
+    trans "N": B <- L^{-1} B (forward); "T" or "C": B <- L^{-T} B (backward).
+    Keyword arguments: partial (default False) skips the last-block and tip
+    solves; device_streaming / buffer raise NotImplementedError as in the
+    reference (pobtas.py:33-36, 49-52) or because the permuted variant is not
+    implemented.
+    """
+    D, L, R, T = L_diagonal_blocks, L_lower_diagonal_blocks, L_lower_arrow_blocks, L_arrow_tip_block
+    n, b, a, dev = bta_shapes(D, L, R, T)
+    _solve(D, L, R, T, B, trans, kwargs, n, b, a, dev, "pobtas")
+
+
+def _solve(D, L, R, T, B, trans, kwargs, n, b, a, dev, what):
+    partial = bool(kwargs.get("partial", False))
+    if kwargs.get("buffer", None) is not None:
+        if kwargs.get("device_streaming", False):
+            raise NotImplementedError("Streaming is not implemented for the permuted arrowhead.")
+        raise NotImplementedError(f"{what}: the permuted (buffer=...) variant is not implemented")
+    if kwargs.get("device_streaming", False):
+        raise NotImplementedError("Streaming is not implemented for the natural arrowhead.")
+    if trans not in ("N", "T", "C"):
+        raise ValueError(f"Invalid transpose argument: {trans}.")
+    _tensor(B, "B")
+    N = n * b + a
+    if B.dim() == 1:
+        r, ldb = 1, 1
+    elif B.dim() == 2:
+        r, ldb = B.shape[1], B.shape[1]
+    else:
+        raise ValueError(f"B must have shape ({N}, r) or ({N},), got {tuple(B.shape)}")
+    if B.shape[0] != N:
+        raise ValueError(f"B must have {N} rows, got {B.shape[0]}")
+    if B.device != dev:
+        raise ValueError("B must be on the same device as the factor")
+    lib = _lib.load()
+    ws, nbytes = workspace(_lib.OP_POBTAS, n, b, a, r, dev)
+    rc = lib.rocbta_dpobtas(n, b, a, ptr(D), ptr(L), ptr(R), ptr(T), ptr(B), r, ldb,
+                            trans.encode(), int(partial), ws.data_ptr(), nbytes, stream_of(dev))
+    _lib.check(rc, what)

@@ -1,0 +1,33 @@
+"""Selected inversion of an SPD BTA matrix (drop-in for serinv.algs.pobtasi).
+
+Reference: src/serinv/algs/pobtasi.py:10-89 (signature, kwargs, dispatch) and
+:92-172 (natural-ordering algorithm); runs as rocbta_dpobtasi.
+"""
+
+from .. import _lib
+from ._common import bta_shapes, check_kwargs, ptr, reject_unsupported, stream_of, workspace
+
+
+def pobtasi(L_diagonal_blocks, L_lower_diagonal_blocks, L_lower_arrow_blocks, L_arrow_tip_block,
+            **kwargs):
+    """Selected inversion from the block Cholesky factor, in place.
+
+    On exit the arrays hold X_ii (full symmetric), X_{i+1,i}, X_{t,i} and X_T,
+    the blocks of A^{-1} on the BTA pattern.  Keyword arguments:
+    invert_last_block (default True); device_streaming / buffer only at their
+    defaults.
+    """
+    check_kwargs(kwargs, {"device_streaming", "buffer", "invert_last_block"})
+    reject_unsupported(kwargs, "pobtasi")
+    invert_last_block = bool(kwargs.get("invert_last_block", True))
+    D, L, R, T = L_diagonal_blocks, L_lower_diagonal_blocks, L_lower_arrow_blocks, L_arrow_tip_block
+    n, b, a, dev = bta_shapes(D, L, R, T)
+    _selinv(D, L, R, T, n, b, a, dev, invert_last_block, "pobtasi")
+
+
+def _selinv(D, L, R, T, n, b, a, dev, invert_last_block, what):
+    lib = _lib.load()
+    ws, nbytes = workspace(_lib.OP_POBTASI, n, b, a, 0, dev)
+    rc = lib.rocbta_dpobtasi(n, b, a, ptr(D), ptr(L), ptr(R), ptr(T), int(invert_last_block),
+                             ws.data_ptr(), nbytes, stream_of(dev))
+    _lib.check(rc, what)

@@ -1,0 +1,582 @@
+// Host orchestration and C ABI of the gfx950 BTA library (librocbta.so).
+//
+// Algorithms (reference recurrences; the kernels evaluate them tile by tile):
+//   pobtaf  : src/serinv/algs/pobtaf.py:97-183 (block Cholesky)
+//   pobtasi : src/serinv/algs/pobtasi.py:92-172 (selected inversion)
+//   pobtas  : src/serinv/algs/pobtas.py:65-161 (triangular solves)
+//   pobtf / pobtsi / pobts: the same with a == 0 (pobtf.py:83-122,
+//   pobtsi.py:78-125, pobts.py:57-118).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdio>
+#include <vector>
+
+#include "../../include/rocbta.h"
+#include "gemm_f64.hpp"
+#include "solve_f64.hpp"
+#include "tile_f64.hpp"
+
+using namespace rocbta;
+
+namespace {
+
+#define RB_CHECK(expr)                                   \
+  do {                                                   \
+    hipError_t e_ = (expr);                              \
+    if (e_ != hipSuccess) {                              \
+      fprintf(stderr, "rocbta: %s failed: %s (%s:%d)\n", #expr, hipGetErrorString(e_), \
+              __FILE__, __LINE__);                       \
+      return ROCBTA_ERR_HIP;                             \
+    }                                                    \
+  } while (0)
+
+inline int cdiv(int64_t x, int64_t y) { return (int)((x + y - 1) / y); }
+inline size_t align256(size_t x) { return (x + 255) & ~size_t(255); }
+
+// ---------------------------------------------------------------- GEMM group
+struct GemmList {
+  std::vector<GemmProblem> probs;
+  GemmList() { probs.reserve(8); }
+
+  GemmProblem& add(double* C, int64_t ldc, int M, int N, double beta, int uplo = TRI_NONE,
+                   int batch1 = 1, int64_t sC1 = 0, int batch2 = 1, int64_t sC2 = 0) {
+    GemmProblem p{};
+    p.C = C;
+    p.ldc = ldc;
+    p.sC1 = sC1;
+    p.sC2 = sC2;
+    p.M = M;
+    p.N = N;
+    p.batch1 = batch1;
+    p.batch2 = batch2;
+    p.beta = beta;
+    p.nterms = 0;
+    p.uplo = uplo;
+    probs.push_back(p);
+    return probs.back();
+  }
+};
+
+GemmTerm term(const double* A, int64_t lda, bool tA, const double* B, int64_t ldb, bool tB, int K,
+              double alpha, int triA = TRI_NONE, int triB = TRI_NONE, int64_t sA1 = 0,
+              int64_t sB1 = 0, int64_t sA2 = 0, int64_t sB2 = 0) {
+  GemmTerm t{};
+  t.A = A;
+  t.B = B;
+  t.lda = lda;
+  t.ldb = ldb;
+  t.sA1 = sA1;
+  t.sA2 = sA2;
+  t.sB1 = sB1;
+  t.sB2 = sB2;
+  t.K = K;
+  t.transA = tA;
+  t.transB = tB;
+  t.triA = (uint8_t)triA;
+  t.triB = (uint8_t)triB;
+  t.alpha = alpha;
+  return t;
+}
+
+void push_term(GemmProblem& p, const GemmTerm& t) {
+  if (t.K <= 0) return;
+  p.t[p.nterms++] = t;
+}
+
+int run_gemms(GemmList& list, hipStream_t s) {
+  GemmGroup g{};
+  auto flush = [&]() -> int {
+    if (g.nprob == 0) return ROCBTA_OK;
+    if (g.total_tiles > 0) {
+      gemm_group_kernel<<<g.total_tiles, GT_THREADS, 0, s>>>(g);
+      RB_CHECK(hipGetLastError());
+    }
+    g = GemmGroup{};
+    return ROCBTA_OK;
+  };
+  for (GemmProblem p : list.probs) {
+    if (p.M <= 0 || p.N <= 0 || p.batch1 <= 0 || p.batch2 <= 0) continue;
+    if (p.nterms == 0 && p.beta == 1.0) continue;
+    p.tiles_m = cdiv(p.M, GT_M);
+    p.tiles_n = cdiv(p.N, GT_N);
+    const int64_t tiles = (int64_t)p.tiles_m * p.tiles_n * p.batch1 * p.batch2;
+    if (g.nprob == ROCBTA_MAX_PROBLEMS || (int64_t)g.total_tiles + tiles > (1ll << 30)) {
+      int rc = flush();
+      if (rc) return rc;
+    }
+    p.tile_begin = g.total_tiles;
+    g.p[g.nprob++] = p;
+    g.total_tiles += (int)tiles;
+  }
+  return flush();
+}
+
+int fill(double* p, int64_t count, double v, hipStream_t s) {
+  if (count <= 0) return ROCBTA_OK;
+  int blocks = (int)std::min<int64_t>(4096, (count + 255) / 256);
+  fill_f64_kernel<<<blocks, 256, 0, s>>>(p, count, v);
+  RB_CHECK(hipGetLastError());
+  return ROCBTA_OK;
+}
+
+// ------------------------------------------------------------ tile kernels
+int tile_factor(int mode, double* A, int64_t lda, int64_t sA1, int64_t sA2, double* X,
+                int64_t ldx, int64_t sX1, int64_t sX2, int m, int batch1, int batch2, int* info,
+                int64_t info_row0, int64_t info_s1, int64_t info_s2, int zcols,
+                hipStream_t s) {
+  if (m <= 0 || batch1 <= 0 || batch2 <= 0) return ROCBTA_OK;
+  TileBatch tb{};
+  tb.A = A;
+  tb.lda = lda;
+  tb.sA1 = sA1;
+  tb.sA2 = sA2;
+  tb.X = X;
+  tb.ldx = ldx;
+  tb.sX1 = sX1;
+  tb.sX2 = sX2;
+  tb.m = m;
+  tb.batch1 = batch1;
+  tb.batch2 = batch2;
+  tb.info = info;
+  tb.info_row0 = info_row0;
+  tb.info_s1 = info_s1;
+  tb.info_s2 = info_s2;
+  tb.zcols = zcols;
+  const int grid = batch1 * batch2;
+  if (mode == 0)
+    tile_factor_kernel<0><<<grid, TT_THREADS, 0, s>>>(tb);
+  else
+    tile_factor_kernel<1><<<grid, TT_THREADS, 0, s>>>(tb);
+  RB_CHECK(hipGetLastError());
+  return ROCBTA_OK;
+}
+
+// ------------------------------------------------ blocked diagonal factor
+struct Panel {
+  double* p;
+  int rows;
+  int64_t ld;
+};
+
+// Cholesky of the m x m diagonal block Dm (row stride ld), with the
+// triangular solves of the panels below it (each rows x m, row stride ld_p)
+// done tile column by tile column: right-looking, 64-wide tiles.
+// tinv: 64 x 64 device scratch for the inverse of the current diagonal tile.
+int factor_block_column(double* Dm, int64_t ld, int m, const std::vector<Panel>& panels,
+                        int64_t row0, int* info, double* tinv, hipStream_t s) {
+  for (int k0 = 0; k0 < m; k0 += TT) {
+    const int tk = std::min(TT, m - k0);
+    const int rem = m - k0 - tk;
+    double* dkk = Dm + (int64_t)k0 * ld + k0;
+    int rc = tile_factor(0, dkk, ld, 0, 0, tinv, TT, 0, 0, tk, 1, 1, info, row0 + k0, 0, 0, rem,
+                         s);
+    if (rc) return rc;
+    // X <- X * inv(L_kk)^T for every row below the diagonal tile (in place;
+    // N = tk <= 64, so each workgroup owns whole rows).
+    GemmList trsm;
+    if (rem > 0) {
+      double* c = Dm + (int64_t)(k0 + tk) * ld + k0;
+      push_term(trsm.add(c, ld, rem, tk, 0.0), term(c, ld, false, tinv, TT, true, tk, 1.0,
+                                                    TRI_NONE, TRI_UPPER));
+    }
+    for (const Panel& pn : panels) {
+      if (pn.rows <= 0) continue;
+      double* c = pn.p + k0;
+      push_term(trsm.add(c, pn.ld, pn.rows, tk, 0.0),
+                term(c, pn.ld, false, tinv, TT, true, tk, 1.0, TRI_NONE, TRI_UPPER));
+    }
+    rc = run_gemms(trsm, s);
+    if (rc) return rc;
+    if (rem <= 0) continue;
+    // right-looking update of the trailing columns
+    GemmList upd;
+    const double* pd = Dm + (int64_t)(k0 + tk) * ld + k0;  // rem x tk
+    push_term(upd.add(Dm + (int64_t)(k0 + tk) * (ld + 1), ld, rem, rem, 1.0, TRI_LOWER),
+              term(pd, ld, false, pd, ld, true, tk, -1.0));
+    for (const Panel& pn : panels) {
+      if (pn.rows <= 0) continue;
+      push_term(upd.add(pn.p + k0 + tk, pn.ld, pn.rows, rem, 1.0),
+                term(pn.p + k0, pn.ld, false, pd, ld, true, tk, -1.0));
+    }
+    rc = run_gemms(upd, s);
+    if (rc) return rc;
+  }
+  return ROCBTA_OK;
+}
+
+// Inverse of `batch` lower-triangular m x m matrices (source stride sL,
+// row stride lds) into X (row stride ldx, stride sX), upper part zeroed.
+// W: scratch of batch * ceil(m/64) * 64 * 64 doubles.
+int block_trtri(const double* Ls, int64_t lds, int64_t sL, double* X, int64_t ldx, int64_t sX,
+                int m, int batch, double* W, hipStream_t s) {
+  if (m <= 0 || batch <= 0) return ROCBTA_OK;
+  const int nt = cdiv(m, TT);
+  const int nfull = m / TT;
+  const int tail = m - nfull * TT;
+  if (ldx != m || sX != (int64_t)m * m) return ROCBTA_ERR_ARG;  // X is a dense stack
+  int rc = fill(X, (int64_t)batch * m * m, 0.0, s);
+  if (rc) return rc;
+  rc = tile_factor(1, const_cast<double*>(Ls), lds, (int64_t)TT * (lds + 1), sL, X, ldx,
+                   (int64_t)TT * (ldx + 1), sX, TT, nfull, batch, nullptr, 0, 0, 0, 0, s);
+  if (rc) return rc;
+  if (tail) {
+    rc = tile_factor(1, const_cast<double*>(Ls) + (int64_t)nfull * TT * (lds + 1), lds, 0, sL,
+                     X + (int64_t)nfull * TT * (ldx + 1), ldx, 0, sX, tail, 1, batch, nullptr, 0,
+                     0, 0, 0, s);
+    if (rc) return rc;
+  }
+  const int64_t wstride = (int64_t)nt * TT * TT;
+  for (int d = 1; d < nt; ++d) {
+    // j = 0 .. nt-1-d ; row tile r = j + d.  Split off the partial last row tile.
+    const int nj = nt - d;
+    const int nj_full = tail ? nj - 1 : nj;
+    GemmList w, x;
+    auto add_range = [&](int j0, int cnt, int Mr) {
+      if (cnt <= 0) return;
+      const int r0 = j0 + d;
+      const double* A = Ls + (int64_t)r0 * TT * lds + (int64_t)j0 * TT;
+      const double* Bx = X + (int64_t)j0 * TT * ldx + (int64_t)j0 * TT;
+      double* Wc = W + (int64_t)j0 * TT * TT;
+      push_term(w.add(Wc, TT, Mr, TT, 0.0, TRI_NONE, cnt, TT * TT, batch, wstride),
+                term(A, lds, false, Bx, ldx, false, d * TT, 1.0, TRI_NONE, TRI_LOWER,
+                     (int64_t)TT * (lds + 1), (int64_t)TT * (ldx + 1), sL, sX));
+      double* Xc = X + (int64_t)r0 * TT * ldx + (int64_t)j0 * TT;
+      const double* Xd = X + (int64_t)r0 * TT * ldx + (int64_t)r0 * TT;
+      push_term(x.add(Xc, ldx, Mr, TT, 0.0, TRI_NONE, cnt, (int64_t)TT * (ldx + 1), batch, sX),
+                term(Xd, ldx, false, Wc, TT, false, Mr, -1.0, TRI_LOWER, TRI_NONE,
+                     (int64_t)TT * (ldx + 1), TT * TT, sX, wstride));
+    };
+    add_range(0, nj_full, TT);
+    if (tail) add_range(nj - 1, 1, tail);
+    rc = run_gemms(w, s);
+    if (rc) return rc;
+    rc = run_gemms(x, s);
+    if (rc) return rc;
+  }
+  return ROCBTA_OK;
+}
+
+// -------------------------------------------------------- workspace layout
+struct SiLayout {
+  size_t linv, g, gt, w, tinv, wt, total;
+};
+
+SiLayout si_layout(int64_t n, int64_t b, int64_t a) {
+  SiLayout l{};
+  size_t off = 0;
+  auto take = [&](size_t doubles) {
+    size_t o = off;
+    off = align256(off + doubles * sizeof(double));
+    return o;
+  };
+  const int64_t ntb = (b + TT - 1) / TT, nta = (a + TT - 1) / TT;
+  l.linv = take((size_t)n * b * b);
+  l.g = take((size_t)std::max<int64_t>(n - 1, 0) * b * b);
+  l.gt = take((size_t)n * a * b);
+  l.w = take((size_t)n * ntb * TT * TT);
+  l.tinv = take((size_t)a * a);
+  l.wt = take((size_t)nta * TT * TT);
+  l.total = off;
+  return l;
+}
+
+bool bad_shape(int64_t n, int64_t b, int64_t a) {
+  return n < 1 || b < 1 || a < 0 || b > (1 << 20) || a > (1 << 20) || n > (1 << 24);
+}
+
+}  // namespace
+
+// ==================================================================== C ABI
+extern "C" {
+
+const char* rocbta_version(void) { return "rocbta 0.1.0 (gfx950, fp64 mfma)"; }
+
+size_t rocbta_workspace_bytes(int op, int64_t n, int64_t b, int64_t a, int64_t r) {
+  (void)r;
+  if (bad_shape(n, b, a)) return 0;
+  switch (op) {
+    case ROCBTA_OP_POBTAF:
+      return align256(TT * TT * sizeof(double));
+    case ROCBTA_OP_POBTASI:
+      return si_layout(n, b, a).total;
+    case ROCBTA_OP_POBTAS:
+      return 256;
+    default:
+      return 0;
+  }
+}
+
+int rocbta_dpobtaf(int64_t n, int64_t b, int64_t a, double* D, double* L, double* R, double* T,
+                   int factorize_last_block, int* info, void* workspace, size_t workspace_bytes,
+                   void* stream) {
+  if (bad_shape(n, b, a) || !D || (n > 1 && !L) || (a > 0 && (!R || !T)))
+    return ROCBTA_ERR_ARG;
+  if (!workspace || workspace_bytes < rocbta_workspace_bytes(ROCBTA_OP_POBTAF, n, b, a, 0))
+    return ROCBTA_ERR_WORKSPACE;
+  hipStream_t s = (hipStream_t)stream;
+  double* tinv = (double*)workspace;
+  if (info) {
+    set_int_kernel<<<1, 64, 0, s>>>(info, 0);
+    RB_CHECK(hipGetLastError());
+  }
+  const int bi = (int)b, ai = (int)a;
+  const int64_t bb = b * b, ab = a * b;
+  for (int64_t i = 0; i + 1 < n; ++i) {
+    double* Di = D + i * bb;
+    std::vector<Panel> panels = {{L + i * bb, bi, b}};
+    if (a > 0) panels.push_back({R + i * ab, ai, b});
+    int rc = factor_block_column(Di, b, bi, panels, i * b, info, tinv, s);
+    if (rc) return rc;
+    // Schur complement update of block row i+1 (pobtaf.py:143-159)
+    const bool full_last = (i + 2 == n) && !factorize_last_block;
+    GemmList upd;
+    const double* Li = L + i * bb;
+    push_term(upd.add(D + (i + 1) * bb, b, bi, bi, 1.0, full_last ? TRI_NONE : TRI_LOWER),
+              term(Li, b, false, Li, b, true, bi, -1.0));
+    if (a > 0) {
+      push_term(upd.add(R + (i + 1) * ab, b, ai, bi, 1.0),
+                term(R + i * ab, b, false, Li, b, true, bi, -1.0));
+      push_term(upd.add(T, a, ai, ai, 1.0), term(R + i * ab, b, false, R + i * ab, b, true, bi, -1.0));
+    }
+    rc = run_gemms(upd, s);
+    if (rc) return rc;
+  }
+  if (factorize_last_block) {  // pobtaf.py:161-183
+    double* Dl = D + (n - 1) * bb;
+    std::vector<Panel> panels;
+    if (a > 0) panels.push_back({R + (n - 1) * ab, ai, b});
+    int rc = factor_block_column(Dl, b, bi, panels, (n - 1) * b, info, tinv, s);
+    if (rc) return rc;
+    if (a > 0) {
+      GemmList upd;
+      const double* Rl = R + (n - 1) * ab;
+      push_term(upd.add(T, a, ai, ai, 1.0), term(Rl, b, false, Rl, b, true, bi, -1.0));
+      rc = run_gemms(upd, s);
+      if (rc) return rc;
+      rc = factor_block_column(T, a, ai, {}, n * b, info, tinv, s);
+      if (rc) return rc;
+    }
+  }
+  return ROCBTA_OK;
+}
+
+int rocbta_dpobtasi(int64_t n, int64_t b, int64_t a, double* D, double* L, double* R, double* T,
+                    int invert_last_block, void* workspace, size_t workspace_bytes,
+                    void* stream) {
+  if (bad_shape(n, b, a) || !D || (n > 1 && !L) || (a > 0 && (!R || !T)))
+    return ROCBTA_ERR_ARG;
+  const SiLayout lay = si_layout(n, b, a);
+  if (!workspace || workspace_bytes < lay.total) return ROCBTA_ERR_WORKSPACE;
+  hipStream_t s = (hipStream_t)stream;
+  char* ws = (char*)workspace;
+  double* Linv = (double*)(ws + lay.linv);
+  double* G = (double*)(ws + lay.g);
+  double* Gt = (double*)(ws + lay.gt);
+  double* W = (double*)(ws + lay.w);
+  double* Tinv = (double*)(ws + lay.tinv);
+  double* Wt = (double*)(ws + lay.wt);
+  const int bi = (int)b, ai = (int)a;
+  const int64_t bb = b * b, ab = a * b;
+  // Blocks whose factor is still stored: all of them, or all but the last
+  // when the caller supplies X_{n-1,n-1}, X_{t,n-1}, X_T (pobtasi.py:114).
+  const int npre = (int)(invert_last_block ? n : n - 1);
+
+  // (1) L_ii^{-1} for every stored block (independent across i)
+  int rc = block_trtri(D, b, bb, Linv, b, bb, bi, npre, W, s);
+  if (rc) return rc;
+  // (2) G_i = L_{i+1,i} L_ii^{-1}, Gt_i = L_{t,i} L_ii^{-1}, S_i = L_ii^{-T} L_ii^{-1} -> D_i
+  {
+    GemmList pre;
+    const int ng = std::min<int>(npre, (int)n - 1);
+    if (ng > 0)
+      push_term(pre.add(G, b, bi, bi, 0.0, TRI_NONE, ng, bb),
+                term(L, b, false, Linv, b, false, bi, 1.0, TRI_NONE, TRI_LOWER, bb, bb));
+    if (a > 0)
+      push_term(pre.add(Gt, b, ai, bi, 0.0, TRI_NONE, npre, ab),
+                term(R, b, false, Linv, b, false, bi, 1.0, TRI_NONE, TRI_LOWER, ab, bb));
+    push_term(pre.add(D, b, bi, bi, 0.0, TRI_NONE, npre, bb),
+              term(Linv, b, true, Linv, b, false, bi, 1.0, TRI_UPPER, TRI_LOWER, bb, bb));
+    rc = run_gemms(pre, s);
+    if (rc) return rc;
+  }
+  if (invert_last_block) {  // pobtasi.py:114-139
+    double* Rl = R + (n - 1) * ab;
+    double* Dl = D + (n - 1) * bb;
+    if (a > 0) {
+      rc = block_trtri(T, a, 0, Tinv, a, a * a, ai, 1, Wt, s);
+      if (rc) return rc;
+      GemmList xt;
+      push_term(xt.add(T, a, ai, ai, 0.0), term(Tinv, a, true, Tinv, a, false, ai, 1.0,
+                                                 TRI_UPPER, TRI_LOWER));
+      rc = run_gemms(xt, s);
+      if (rc) return rc;
+      GemmList xr;
+      push_term(xr.add(Rl, b, ai, bi, 0.0),
+                term(T, a, false, Gt + (n - 1) * ab, b, false, ai, -1.0));
+      rc = run_gemms(xr, s);
+      if (rc) return rc;
+      GemmList xd;
+      push_term(xd.add(Dl, b, bi, bi, 1.0),
+                term(Rl, b, true, Gt + (n - 1) * ab, b, false, ai, -1.0));
+      rc = run_gemms(xd, s);
+      if (rc) return rc;
+    }
+  }
+  // (3) backward recurrence (pobtasi.py:141-172) with the precomputed G's
+  for (int64_t i = n - 2; i >= 0; --i) {
+    GemmList off;
+    GemmProblem& pl = off.add(L + i * bb, b, bi, bi, 0.0);
+    push_term(pl, term(D + (i + 1) * bb, b, false, G + i * bb, b, false, bi, -1.0));
+    if (a > 0) {
+      push_term(pl, term(R + (i + 1) * ab, b, true, Gt + i * ab, b, false, ai, -1.0));
+      GemmProblem& pr = off.add(R + i * ab, b, ai, bi, 0.0);
+      push_term(pr, term(R + (i + 1) * ab, b, false, G + i * bb, b, false, bi, -1.0));
+      push_term(pr, term(T, a, false, Gt + i * ab, b, false, ai, -1.0));
+    }
+    rc = run_gemms(off, s);
+    if (rc) return rc;
+    GemmList dia;
+    GemmProblem& pd = dia.add(D + i * bb, b, bi, bi, 1.0);
+    push_term(pd, term(L + i * bb, b, true, G + i * bb, b, false, bi, -1.0));
+    if (a > 0) push_term(pd, term(R + i * ab, b, true, Gt + i * ab, b, false, ai, -1.0));
+    rc = run_gemms(dia, s);
+    if (rc) return rc;
+  }
+  return ROCBTA_OK;
+}
+
+}  // extern "C"
+
+namespace {
+
+int trsv_tile(const double* Lt, int64_t ld, int m, double* B, int64_t ldb, int r, int trans,
+              hipStream_t s) {
+  if (m <= 0 || r <= 0) return ROCBTA_OK;
+  trsv_tile_kernel<<<cdiv(r, SV_COLS), SV_THREADS, 0, s>>>(Lt, ld, m, B, ldb, r, trans);
+  RB_CHECK(hipGetLastError());
+  return ROCBTA_OK;
+}
+
+// Forward solve of one diagonal block (m x m, stride ld) on Bi (m x r), with
+// the right-looking updates of `below` panels: {matrix (rows x m), target B}.
+struct Below {
+  const double* M;
+  int64_t ld;
+  int rows;
+  double* Bt;
+};
+
+int forward_block(const double* Dm, int64_t ld, int m, double* Bi, int64_t ldb, int r,
+                  const std::vector<Below>& below, hipStream_t s) {
+  for (int k0 = 0; k0 < m; k0 += TT) {
+    const int tk = std::min(TT, m - k0);
+    int rc = trsv_tile(Dm + (int64_t)k0 * (ld + 1), ld, tk, Bi + (int64_t)k0 * ldb, ldb, r, 0, s);
+    if (rc) return rc;
+    GemmList upd;
+    const double* yk = Bi + (int64_t)k0 * ldb;
+    const int rem = m - k0 - tk;
+    if (rem > 0)
+      push_term(upd.add(Bi + (int64_t)(k0 + tk) * ldb, ldb, rem, r, 1.0),
+                term(Dm + (int64_t)(k0 + tk) * ld + k0, ld, false, yk, ldb, false, tk, -1.0));
+    for (const Below& bl : below)
+      if (bl.rows > 0)
+        push_term(upd.add(bl.Bt, ldb, bl.rows, r, 1.0),
+                  term(bl.M + k0, bl.ld, false, yk, ldb, false, tk, -1.0));
+    rc = run_gemms(upd, s);
+    if (rc) return rc;
+  }
+  return ROCBTA_OK;
+}
+
+int backward_block(const double* Dm, int64_t ld, int m, double* Bi, int64_t ldb, int r,
+                   hipStream_t s) {
+  const int nt = cdiv(m, TT);
+  for (int t = nt - 1; t >= 0; --t) {
+    const int k0 = t * TT, tk = std::min(TT, m - k0);
+    int rc = trsv_tile(Dm + (int64_t)k0 * (ld + 1), ld, tk, Bi + (int64_t)k0 * ldb, ldb, r, 1, s);
+    if (rc) return rc;
+    if (k0 == 0) continue;
+    GemmList upd;
+    push_term(upd.add(Bi, ldb, k0, r, 1.0),
+              term(Dm + (int64_t)k0 * ld, ld, true, Bi + (int64_t)k0 * ldb, ldb, false, tk, -1.0));
+    rc = run_gemms(upd, s);
+    if (rc) return rc;
+  }
+  return ROCBTA_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int rocbta_dpobtas(int64_t n, int64_t b, int64_t a, const double* D, const double* L,
+                   const double* R, const double* T, double* B, int64_t r, int64_t ldb,
+                   char trans, int partial, void* workspace, size_t workspace_bytes,
+                   void* stream) {
+  (void)workspace;
+  (void)workspace_bytes;
+  if (bad_shape(n, b, a) || !D || !B || (n > 1 && !L) || (a > 0 && (!R || !T)) || r < 0 ||
+      ldb < r)
+    return ROCBTA_ERR_ARG;
+  if (trans != 'N' && trans != 'T' && trans != 'C') return ROCBTA_ERR_ARG;
+  if (r == 0) return ROCBTA_OK;
+  hipStream_t s = (hipStream_t)stream;
+  const int bi = (int)b, ai = (int)a, ri = (int)r;
+  const int64_t bb = b * b, ab = a * b;
+  double* Bt = B + n * b * ldb;  // tip rows
+  if (trans == 'N') {  // pobtas.py:80-126
+    for (int64_t i = 0; i + 1 < n; ++i) {
+      std::vector<Below> below = {{L + i * bb, b, bi, B + (i + 1) * b * ldb}};
+      if (a > 0) below.push_back({R + i * ab, b, ai, Bt});
+      int rc = forward_block(D + i * bb, b, bi, B + i * b * ldb, ldb, ri, below, s);
+      if (rc) return rc;
+    }
+    if (!partial) {
+      std::vector<Below> below;
+      if (a > 0) below.push_back({R + (n - 1) * ab, b, ai, Bt});
+      int rc = forward_block(D + (n - 1) * bb, b, bi, B + (n - 1) * b * ldb, ldb, ri, below, s);
+      if (rc) return rc;
+      if (a > 0) {
+        rc = forward_block(T, a, ai, Bt, ldb, ri, {}, s);
+        if (rc) return rc;
+      }
+    }
+  } else {  // pobtas.py:127-159
+    if (!partial) {
+      if (a > 0) {
+        int rc = backward_block(T, a, ai, Bt, ldb, ri, s);
+        if (rc) return rc;
+        GemmList upd;
+        push_term(upd.add(B + (n - 1) * b * ldb, ldb, bi, ri, 1.0),
+                  term(R + (n - 1) * ab, b, true, Bt, ldb, false, ai, -1.0));
+        rc = run_gemms(upd, s);
+        if (rc) return rc;
+      }
+      int rc = backward_block(D + (n - 1) * bb, b, bi, B + (n - 1) * b * ldb, ldb, ri, s);
+      if (rc) return rc;
+    }
+    for (int64_t i = n - 2; i >= 0; --i) {
+      GemmList upd;
+      GemmProblem& p = upd.add(B + i * b * ldb, ldb, bi, ri, 1.0);
+      push_term(p, term(L + i * bb, b, true, B + (i + 1) * b * ldb, ldb, false, bi, -1.0));
+      if (a > 0) push_term(p, term(R + i * ab, b, true, Bt, ldb, false, ai, -1.0));
+      int rc = run_gemms(upd, s);
+      if (rc) return rc;
+      rc = backward_block(D + i * bb, b, bi, B + i * b * ldb, ldb, ri, s);
+      if (rc) return rc;
+    }
+  }
+  return ROCBTA_OK;
+}
+
+int rocbta_dlogdet(int64_t n, int64_t b, int64_t a, const double* D, const double* T,
+                   double* out, void* stream) {
+  if (bad_shape(n, b, a) || !D || !out || (a > 0 && !T)) return ROCBTA_ERR_ARG;
+  hipStream_t s = (hipStream_t)stream;
+  logdet_kernel<<<1, 256, 0, s>>>(D, n, b, T, a, out);
+  RB_CHECK(hipGetLastError());
+  return ROCBTA_OK;
+}
+
+}  // extern "C"

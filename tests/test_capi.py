@@ -1,0 +1,58 @@
+"""The C-ABI library loads and exports every symbol the header declares (CPU).
+
+No compute entry point is called here; only host-side argument checks,
+which return before any device work.
+"""
+
+import ctypes
+import os
+import re
+
+import pytest
+
+from serinv_amd import _lib
+
+HEADER = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "include", "rocbta.h")
+
+
+def declared_symbols():
+    text = open(HEADER).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(rocbta_\w+)\s*\(", text)))
+
+
+@pytest.fixture(scope="module")
+def lib():
+    if not os.path.exists(_lib.LIB_PATH):
+        pytest.fail(f"{_lib.LIB_PATH} not built (run __graft_entry__.build())")
+    return _lib.load()
+
+
+def test_header_symbols_exported(lib):
+    syms = declared_symbols()
+    assert "rocbta_dpobtaf" in syms and "rocbta_dpobtasi" in syms and "rocbta_dpobtas" in syms
+    for s in syms:
+        assert hasattr(lib, s), f"{s} declared in rocbta.h but not exported"
+        assert s in _lib.SIGNATURES, f"{s} has no ctypes signature"
+
+
+def test_version(lib):
+    assert b"gfx950" in lib.rocbta_version()
+
+
+def test_workspace_sizes(lib):
+    n, b, a = 256, 1024, 32
+    assert lib.rocbta_workspace_bytes(_lib.OP_POBTAF, n, b, a, 0) >= 64 * 64 * 8
+    si = lib.rocbta_workspace_bytes(_lib.OP_POBTASI, n, b, a, 0)
+    assert si >= (2 * n - 1) * b * b * 8  # L^{-1} blocks and G blocks
+    assert lib.rocbta_workspace_bytes(_lib.OP_POBTAF, 0, b, a, 0) == 0  # invalid shape
+    assert lib.rocbta_workspace_bytes(99, n, b, a, 0) == 0
+
+
+def test_argument_errors_return_before_launch(lib):
+    # null diagonal blocks / negative sizes are rejected on the host
+    assert lib.rocbta_dpobtaf(4, 8, 2, None, None, None, None, 1, None, None, 0, None) == _lib.ROCBTA_ERR_ARG
+    assert lib.rocbta_dpobtasi(0, 8, 2, None, None, None, None, 1, None, 0, None) == _lib.ROCBTA_ERR_ARG
+    fake = ctypes.c_void_p(16)
+    assert lib.rocbta_dpobtas(2, 4, 1, fake, fake, fake, fake, fake, 1, 1, b"X", 0, None, 0, None) == _lib.ROCBTA_ERR_ARG
+    assert lib.rocbta_dpobtaf(2, 4, 1, fake, fake, fake, fake, 1, None, None, 0, None) == _lib.ROCBTA_ERR_WORKSPACE

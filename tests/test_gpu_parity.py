@@ -1,0 +1,186 @@
+"""GPU parity: the HIP path (through librocbta.so) against the oracle and the
+dense-LAPACK golden vectors, on the same seeded inputs.
+
+Tolerance: 1e-12 relative (max-norm over each output array), the FP64 bar
+of the drop-in.  Shapes cover the reference test grid (n 1..4, b and a in
+{2,3}, r in {1,2,3}; tests/tests_algs/regular/...), 64-wide tile
+boundaries (b, a in {1, 63, 64, 65, 100, 128, 130}), the arrowless case
+and the n=32, b=64, a=16 workload.
+"""
+
+import numpy as np
+import pytest
+
+from oracle import dense, generators, serinv_np
+from serinv_amd.utils.matrices import bta_spd
+from tests._helpers import golden_cases, relerr, to_dev, to_host
+
+pytestmark = pytest.mark.gpu
+TOL = 1e-12
+
+
+@pytest.fixture(scope="module")
+def algs(gpu_device):
+    import serinv_amd.algs as A
+
+    return A
+
+
+def oracle_all(D, L, R, T, B):
+    D, L, R, T, B = (x.copy() for x in (D, L, R, T, B))
+    serinv_np.pobtaf(D, L, R, T)
+    fac = [x.copy() for x in (D, L, R, T)]
+    ld = serinv_np.logdet_from_factor(D, T)
+    serinv_np.pobtas(D, L, R, T, B, trans="N")
+    y = B.copy()
+    serinv_np.pobtas(D, L, R, T, B, trans="C")
+    serinv_np.pobtasi(D, L, R, T)
+    return fac, ld, y, B, [D, L, R, T]
+
+
+def gpu_all(algs, dev, D, L, R, T, B):
+    from serinv_amd.logdet import pobta_logdet
+
+    d, l, r, t, bb = to_dev(dev, D, L, R, T, B)
+    algs.pobtaf(d, l, r, t)
+    fac = to_host(d, l, r, t)
+    ld = float(pobta_logdet(d, t).item())
+    algs.pobtas(d, l, r, t, bb, trans="N")
+    y = to_host(bb)[0]
+    algs.pobtas(d, l, r, t, bb, trans="C")
+    x = to_host(bb)[0]
+    algs.pobtasi(d, l, r, t)
+    return fac, ld, y, x, to_host(d, l, r, t)
+
+
+def compare(got, ref, tag=""):
+    fac_g, ld_g, y_g, x_g, X_g = got
+    fac_r, ld_r, y_r, x_r, X_r = ref
+    names = ("D", "L", "R", "T")
+    for nm, g, r in zip(names, fac_g, fac_r):
+        assert relerr(g, r) < TOL, f"{tag} factor {nm}: {relerr(g, r):.3e}"
+    assert abs(ld_g - ld_r) <= TOL * max(1.0, abs(ld_r)), f"{tag} logdet {ld_g} vs {ld_r}"
+    assert relerr(y_g, y_r) < TOL, f"{tag} forward solve: {relerr(y_g, y_r):.3e}"
+    assert relerr(x_g, x_r) < TOL, f"{tag} solve: {relerr(x_g, x_r):.3e}"
+    for nm, g, r in zip(names, X_g, X_r):
+        assert relerr(g, r) < TOL, f"{tag} selected inverse {nm}: {relerr(g, r):.3e}"
+    # scipy-style zero upper triangle of the factor
+    assert np.all(np.triu(fac_g[0], 1) == 0) and np.all(np.triu(fac_g[3], 1) == 0)
+
+
+REF_GRID = [(b, a, n, r) for b in (2, 3) for a in (2, 3) for n in (1, 2, 3, 4) for r in (1, 3)]
+
+
+@pytest.mark.parametrize("b,a,n,r", REF_GRID)
+def test_reference_grid(algs, gpu_device, b, a, n, r):
+    rng = np.random.default_rng(10_000 + 1000 * b + 100 * a + 10 * n + r)
+    A = generators.dd_bta(b, a, n, rng)
+    generators.symmetrize(A)
+    B = generators.rhs(r, b, a, n, rng)
+    D, L, R, T = dense.bta_dense_to_arrays(A, b, a, n)
+    compare(gpu_all(algs, gpu_device, D, L, R, T, B), oracle_all(D, L, R, T, B), f"b{b}a{a}n{n}r{r}")
+
+
+TILE_SHAPES = [(3, 64, 16, 2), (2, 65, 1, 1), (4, 100, 70, 3), (2, 128, 64, 5), (3, 130, 0, 2),
+               (1, 200, 33, 1), (5, 63, 63, 64), (2, 1, 1, 1), (6, 8, 130, 70)]
+
+
+@pytest.mark.parametrize("n,b,a,r", TILE_SHAPES)
+def test_tile_boundaries(algs, gpu_device, n, b, a, r):
+    D, L, R, T, B = bta_spd(n, b, a, seed=500 + n + b + a, n_rhs=r)
+    compare(gpu_all(algs, gpu_device, D, L, R, T, B), oracle_all(D, L, R, T, B), f"n{n}b{b}a{a}")
+
+
+def test_workload_n32_b64_a16(algs, gpu_device):
+    D, L, R, T, B = bta_spd(32, 64, 16, n_rhs=1)
+    compare(gpu_all(algs, gpu_device, D, L, R, T, B), oracle_all(D, L, R, T, B), "n32b64a16")
+
+
+@pytest.mark.parametrize("name", sorted(golden_cases()))
+def test_golden_vectors(algs, gpu_device, name):
+    c = golden_cases()[name]
+    got = gpu_all(algs, gpu_device, c["D"], c["L"], c["R"], c["T"], c["B"])
+    ref = ([c["Lf_D"], c["Lf_L"], c["Lf_R"], c["Lf_T"]], float(c["logdet"]), c["Y"], c["Xs"],
+           [c["X_D"], c["X_L"], c["X_R"], c["X_T"]])
+    compare(got, ref, name)
+
+
+@pytest.mark.parametrize("n,b", [(1, 3), (4, 2), (3, 70), (5, 64)])
+def test_bt_routines(algs, gpu_device, n, b):
+    D, L, _, _, B = bta_spd(n, b, 0, seed=77 + n, n_rhs=2)
+    Do, Lo, Bo = D.copy(), L.copy(), B.copy()
+    serinv_np.pobtf(Do, Lo)
+    d, l, bb = to_dev(gpu_device, D, L, B)
+    algs.pobtf(d, l)
+    assert relerr(to_host(d)[0], Do) < TOL and relerr(to_host(l)[0], Lo) < TOL
+    serinv_np.pobts(Do, Lo, Bo, trans="N")
+    serinv_np.pobts(Do, Lo, Bo, trans="C")
+    algs.pobts(d, l, bb, trans="N")
+    algs.pobts(d, l, bb, trans="C")
+    assert relerr(to_host(bb)[0], Bo) < TOL
+    serinv_np.pobtsi(Do, Lo)
+    algs.pobtsi(d, l)
+    assert relerr(to_host(d)[0], Do) < TOL and relerr(to_host(l)[0], Lo) < TOL
+
+
+@pytest.mark.parametrize("n,b,a", [(4, 3, 2), (3, 70, 5), (2, 64, 0)])
+def test_partial_flags(algs, gpu_device, n, b, a):
+    D, L, R, T, B = bta_spd(n, b, a, seed=900 + n, n_rhs=2)
+    # factorize_last_block=False
+    Do, Lo, Ro, To = (x.copy() for x in (D, L, R, T))
+    serinv_np.pobtaf(Do, Lo, Ro, To, factorize_last_block=False)
+    d, l, r, t = to_dev(gpu_device, D, L, R, T)
+    algs.pobtaf(d, l, r, t, factorize_last_block=False)
+    for g, o in zip(to_host(d, l, r, t), (Do, Lo, Ro, To)):
+        assert relerr(g, o) < TOL
+    # invert_last_block=False on top of a full factorization with true X blocks
+    Do, Lo, Ro, To = (x.copy() for x in (D, L, R, T))
+    serinv_np.pobtaf(Do, Lo, Ro, To)
+    Xd, Xl, Xr, Xt = dense.bta_dense_to_arrays(np.linalg.inv(dense.bta_arrays_to_dense(D, L, R, T)), b, a, n)
+    Do[-1], Ro[-1], To[:] = Xd[-1], Xr[-1], Xt
+    d, l, r, t = to_dev(gpu_device, Do, Lo, Ro, To)
+    serinv_np.pobtasi(Do, Lo, Ro, To, invert_last_block=False)
+    algs.pobtasi(d, l, r, t, invert_last_block=False)
+    for g, o in zip(to_host(d, l, r, t), (Do, Lo, Ro, To)):
+        assert relerr(g, o) < TOL
+    # partial solves
+    Do, Lo, Ro, To = (x.copy() for x in (D, L, R, T))
+    serinv_np.pobtaf(Do, Lo, Ro, To)
+    d, l, r, t, bb = to_dev(gpu_device, Do, Lo, Ro, To, B)
+    Bo = B.copy()
+    for trans in ("N", "C"):
+        serinv_np.pobtas(Do, Lo, Ro, To, Bo, trans=trans, partial=True)
+        algs.pobtas(d, l, r, t, bb, trans=trans, partial=True)
+        assert relerr(to_host(bb)[0], Bo) < TOL
+
+
+def test_vector_rhs(algs, gpu_device):
+    D, L, R, T, B = bta_spd(3, 10, 4, seed=3, n_rhs=1)
+    Do, Lo, Ro, To = (x.copy() for x in (D, L, R, T))
+    serinv_np.pobtaf(Do, Lo, Ro, To)
+    b1 = B[:, 0].copy()
+    serinv_np.pobtas(Do, Lo, Ro, To, b1, trans="N")
+    d, l, r, t, bb = to_dev(gpu_device, Do, Lo, Ro, To, B[:, 0].copy())
+    algs.pobtas(d, l, r, t, bb, trans="N")
+    assert relerr(to_host(bb)[0], b1) < TOL
+
+
+def test_errors(algs, gpu_device):
+    D, L, R, T, B = bta_spd(3, 8, 2, n_rhs=1)
+    d, l, r, t, bb = to_dev(gpu_device, D, L, R, T, B)
+    with pytest.raises(TypeError):
+        algs.pobtaf(d, l, r, t, bogus=True)
+    with pytest.raises(TypeError):
+        algs.pobtasi(d, l, r, t, bogus=True)
+    with pytest.raises(ValueError):
+        algs.pobtas(d, l, r, t, bb, trans="X")
+    with pytest.raises(NotImplementedError):
+        algs.pobtas(d, l, r, t, bb, device_streaming=True)
+    with pytest.raises(TypeError):
+        algs.pobtaf(*to_dev("cpu", D, L, R, T))
+    Dbad = D.copy()
+    Dbad[1] = -np.eye(8)
+    d, l, r, t = to_dev(gpu_device, Dbad, L, R, T)
+    with pytest.raises(np.linalg.LinAlgError) as ei:
+        algs.pobtaf(d, l, r, t)
+    assert "order 9 " in str(ei.value)  # first pivot of block 1 = global row 9
