@@ -1,0 +1,250 @@
+"""Benchmark: FP64 pobtaf + pobtasi on one MI355X (n=256, b=1024, a=32).
+
+One step = restore the input blocks (device-to-device copy, untimed by the
+events) + pobtaf + pobtasi on the seeded SPD BTA matrix, all resident in HBM.
+`value` is the HIP-event time of pobtaf + pobtasi per step (mean over the
+timed steps), in ms.  ms_per_step is the wall time per step of the bracketed
+region (restore copy included).  pobtas (1 and 64 RHS) is timed separately.
+
+The CPU baseline is the NumPy restatement of the reference algorithm
+(oracle/serinv_np.py) timed on this host's cores on a sample of the same
+workload (the first n_s block rows, b=1024, a=32), scaled to n=256 by the
+algorithmic FLOP ratio.
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--n N --b B --a A]
+                       [--no-cpu-baseline] [--profile-steps K]
+"""
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+FP64_PEAK_TFLOPS = 78.6  # MI355X dense FP64 matrix peak (datasheet)
+HBM_PEAK_GBS = 8000.0
+METRIC = "pobtaf+pobtasi ms & %FP64-MFMA peak; mixed/batched speedup vs plain at stated err"
+
+
+def parse():
+    p = argparse.ArgumentParser()
+    p.add_argument("--gpus", type=int, default=1)
+    p.add_argument("--steps", type=int, default=5)
+    p.add_argument("--warmup", type=int, default=1)
+    p.add_argument("--n", type=int, default=256)
+    p.add_argument("--b", type=int, default=1024)
+    p.add_argument("--a", type=int, default=32)
+    p.add_argument("--gamma", type=float, default=None,
+                   help="diagonal-shift scale (default: calibrated value for kappa ~ 1e6)")
+    p.add_argument("--no-cpu-baseline", action="store_true")
+    p.add_argument("--cpu-seconds", type=float, default=15.0)
+    p.add_argument("--no-solve", action="store_true")
+    return p.parse_args()
+
+
+def calibrated_gamma(n, b, a):
+    path = os.path.join(ROOT, "serinv_amd", "utils", "calibration.json")
+    try:
+        with open(path) as f:
+            table = json.load(f)
+        return float(table[f"{n},{b},{a},1e6"]["gamma"]), "kappa~1e6 (calibrated)"
+    except (OSError, KeyError, ValueError):
+        return 1.0, "gamma=1 (strictly diagonally dominant)"
+
+
+def cpu_baseline(b, a, n_full, budget_s):
+    """Time the NumPy restatement on the first n_s block rows; scale by FLOPs."""
+    import numpy as np
+    from threadpoolctl import threadpool_info, threadpool_limits
+
+    from oracle import serinv_np
+    from serinv_amd.utils.matrices import bta_flops, bta_spd
+
+    cores = len(os.sched_getaffinity(0))
+    with threadpool_limits(limits=cores):
+        info = threadpool_info()
+        threads = max([i.get("num_threads", 1) for i in info] or [1])
+
+        def run(ns):
+            D, L, R, T, _ = bta_spd(ns, b, a, seed=63)
+            t0 = time.perf_counter()
+            serinv_np.pobtaf(D, L, R, T)
+            t1 = time.perf_counter()
+            serinv_np.pobtasi(D, L, R, T)
+            t2 = time.perf_counter()
+            return t1 - t0, t2 - t1
+
+        run(2)  # warm-up
+        ns = 4
+        taf, tsi = run(ns)
+        per = (taf + tsi) / max(sum(bta_flops(ns, b, a)), 1.0)
+        ns2 = int(min(64, max(4, budget_s / max(per * sum(bta_flops(2, b, a)) / 2, 1e-9))))
+        if ns2 > ns:
+            ns = ns2
+            taf, tsi = run(ns)
+    f_s = sum(bta_flops(ns, b, a))
+    f_full = sum(bta_flops(n_full, b, a))
+    ms_full = (taf + tsi) * f_full / f_s * 1e3
+    blas = ",".join(sorted({f"{i.get('internal_api')} {i.get('version')}" for i in info}))
+    return {
+        "value": round(ms_full, 1),
+        "unit": "ms (pobtaf+pobtasi, scaled to the full workload)",
+        "cores": threads,
+        "kind": "port",
+        "sample": f"oracle/serinv_np.py pobtaf+pobtasi on n={ns} b={b} a={a} "
+                  f"({taf + tsi:.2f} s measured: pobtaf {taf:.2f} s, pobtasi {tsi:.2f} s), "
+                  f"scaled by FLOP ratio {f_full / f_s:.2f}; BLAS {blas}",
+        "tflops": round(f_s / (taf + tsi) * 1e-12, 4),
+    }
+
+
+def main():
+    args = parse()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py measures one GPU only: the BTA factorization path here is single-GPU "
+              "(see DESIGN.md for how it would shard); run with --gpus 1.", file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+
+    import serinv_amd.algs as algs
+    from serinv_amd.algs import config as algs_config
+    from serinv_amd.utils.matrices import (bta_factor_bytes, bta_flops, bta_solve_flops,
+                                           bta_spd)
+
+    dev = torch.device("cuda:0")
+    n, b, a = args.n, args.b, args.a
+    gamma, cond_note = (args.gamma, f"gamma={args.gamma}") if args.gamma else calibrated_gamma(n, b, a)
+    t0 = time.time()
+    D, L, R, T, Bm = bta_spd(n, b, a, seed=63, gamma=gamma, n_rhs=64)
+    print(f"[bench] generated n={n} b={b} a={a} gamma={gamma:g} in {time.time() - t0:.1f}s",
+          file=sys.stderr, flush=True)
+    D0, L0, R0, T0 = (torch.from_numpy(x).to(dev) for x in (D, L, R, T))
+    B0 = torch.from_numpy(Bm).to(dev)
+    del D, L, R, T
+    Dw, Lw, Rw, Tw = (torch.empty_like(x) for x in (D0, L0, R0, T0))
+    stream = torch.cuda.current_stream(dev)
+    algs_config["check_info"] = False
+
+    def restore():
+        for w, o in ((Dw, D0), (Lw, L0), (Rw, R0), (Tw, T0)):
+            w.copy_(o)
+
+    ev = [[torch.cuda.Event(enable_timing=True) for _ in range(3)] for _ in range(args.steps)]
+
+    def step(k=None):
+        restore()
+        if k is not None:
+            ev[k][0].record(stream)
+        algs.pobtaf(Dw, Lw, Rw, Tw)
+        if k is not None:
+            ev[k][1].record(stream)
+        algs.pobtasi(Dw, Lw, Rw, Tw)
+        if k is not None:
+            ev[k][2].record(stream)
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    print("[bench] warmup done", file=sys.stderr, flush=True)
+    t_start = time.perf_counter()
+    for k in range(args.steps):
+        step(k)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t_start
+    t_af = [ev[k][0].elapsed_time(ev[k][1]) for k in range(args.steps)]
+    t_si = [ev[k][1].elapsed_time(ev[k][2]) for k in range(args.steps)]
+    t_tot = [x + y for x, y in zip(t_af, t_si)]
+    ms = float(np.mean(t_tot))
+    f_af, f_si = bta_flops(n, b, a)
+    tflops = (f_af + f_si) / (ms * 1e-3) * 1e-12
+    print(f"[bench] pobtaf {np.mean(t_af):.2f} ms  pobtasi {np.mean(t_si):.2f} ms  "
+          f"total {ms:.2f} ms  {tflops:.2f} TFLOP/s", file=sys.stderr, flush=True)
+
+    # Full-size property check on the last step's output: X_T from the
+    # selected inversion against the tip of the solution A^{-1} E_t.
+    extra = {}
+    if not args.no_solve:
+        algs_config["check_info"] = True
+        restore()
+        algs.pobtaf(Dw, Lw, Rw, Tw)
+        torch.cuda.synchronize()
+        sv = {}
+        for r in (1, 64):
+            Bw = B0[:, :r].contiguous().clone()
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record(stream)
+            algs.pobtas(Dw, Lw, Rw, Tw, Bw, trans="N")
+            algs.pobtas(Dw, Lw, Rw, Tw, Bw, trans="C")
+            e1.record(stream)
+            torch.cuda.synchronize()
+            t = e0.elapsed_time(e1)
+            sv[r] = (t, Bw)
+            extra[f"pobtas_ms_r{r}"] = round(t, 3)
+        byts = 2 * bta_factor_bytes(n, b, a)
+        extra["pobtas_GBs_r1"] = round(byts / (sv[1][0] * 1e-3) * 1e-9, 1)
+        # residual || A x - B || / ||B|| via block matvec on the original blocks
+        x = sv[64][1]
+        Ax = torch.zeros_like(x)
+        xb = x[: n * b].view(n, b, -1)
+        Ax[: n * b].view(n, b, -1).copy_(torch.bmm(D0, xb))
+        if n > 1:
+            Ax[b: n * b].view(n - 1, b, -1).add_(torch.bmm(L0, xb[:-1]))
+            Ax[: (n - 1) * b].view(n - 1, b, -1).add_(torch.bmm(L0.transpose(1, 2).contiguous(), xb[1:]))
+        if a:
+            xt = x[n * b:]
+            Rflat = R0.permute(1, 0, 2).contiguous().view(a, n * b)
+            Ax[n * b:] += Rflat @ x[: n * b] + T0 @ xt
+            Ax[: n * b].view(n, b, -1).add_(torch.bmm(R0.transpose(1, 2).contiguous(), xt.expand(n, a, -1).contiguous()))
+        res = float((Ax - B0).norm() / B0.norm())
+        extra["solve_rel_residual_r64"] = float(f"{res:.3e}")
+        print(f"[bench] pobtas r=1 {extra['pobtas_ms_r1']} ms, r=64 {extra['pobtas_ms_r64']} ms, "
+              f"residual {res:.2e}", file=sys.stderr, flush=True)
+
+    roofline = {
+        "bound": "mfma",
+        "achieved": round(tflops, 3),
+        "peak": FP64_PEAK_TFLOPS,
+        "unit": "TFLOP/s",
+        "frac": round(tflops / FP64_PEAK_TFLOPS, 4),
+        "traffic": None,
+        "kernel": "pobtaf+pobtasi launch sequence (algorithmic FLOPs of the reference recurrence)",
+    }
+    out = {
+        "metric": METRIC,
+        "value": round(ms, 3),
+        "unit": "ms",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(wall / args.steps * 1e3, 3),
+        "higher_is_better": False,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": f"synthetic seeded SPD BTA (seed 63, {cond_note})",
+        "config": {"workload": f"FP64 pobtaf+pobtasi n={n} b={b} a={a}", "n": n, "b": b, "a": a,
+                   "gamma": gamma},
+        "pobtaf_ms": round(float(np.mean(t_af)), 3),
+        "pobtasi_ms": round(float(np.mean(t_si)), 3),
+        "tflops": round(tflops, 3),
+        "frac_fp64_peak": round(tflops / FP64_PEAK_TFLOPS, 4),
+        "mixed_speedup": None,
+        "batched_speedup": None,
+        "roofline": roofline,
+        **extra,
+    }
+    if not args.no_cpu_baseline:
+        print("[bench] timing the CPU baseline ...", file=sys.stderr, flush=True)
+        out["cpu_baseline"] = cpu_baseline(b, a, n, args.cpu_seconds)
+    else:
+        out["cpu_baseline"] = None
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

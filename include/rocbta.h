@@ -78,6 +78,14 @@ int rocbta_dpobtas(int64_t n, int64_t b, int64_t a, const double* D, const doubl
 int rocbta_dlogdet(int64_t n, int64_t b, int64_t a, const double* D, const double* T,
                    double* out, void* stream);
 
+/* Diagnostic (no counterpart in the reference): runs the task decoding and
+ * dependency enumeration of the persistent pobtaf kernel on the HOST for the
+ * given shape and returns the number of violations (a task waiting on a task
+ * that is not scheduled strictly before it, a task index that does not
+ * decode back to itself, a tile outside its array); 0 means the schedule is
+ * deadlock-free and in bounds.  -1: shape not handled by that kernel. */
+int64_t rocbta_dag_selfcheck(int64_t n, int64_t b, int64_t a);
+
 #ifdef __cplusplus
 }
 #endif

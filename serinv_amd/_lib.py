@@ -39,6 +39,7 @@ SIGNATURES = {
     "rocbta_dpobtas": (_c_int, [_c_i64, _c_i64, _c_i64, _c_p, _c_p, _c_p, _c_p, _c_p, _c_i64,
                                 _c_i64, ctypes.c_char, _c_int, _c_p, _c_size, _c_p]),
     "rocbta_dlogdet": (_c_int, [_c_i64, _c_i64, _c_i64, _c_p, _c_p, _c_p, _c_p]),
+    "rocbta_dag_selfcheck": (_c_i64, [_c_i64, _c_i64, _c_i64]),
 }
 
 _lib = None

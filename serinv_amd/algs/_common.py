@@ -91,6 +91,8 @@ def raise_on_info(info, what):
     if not config["check_info"]:
         return
     v = int(info.item())
+    if v < 0:
+        raise _lib.RocbtaError(f"{what}: internal scheduling time-out in the factorization kernel")
     if v != 0:
         # numpy/scipy Cholesky raise numpy.linalg.LinAlgError (the reference's
         # host path, src/serinv/__init__.py:151)

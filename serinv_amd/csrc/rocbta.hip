@@ -14,8 +14,12 @@
 
 #include "../../include/rocbta.h"
 #include "gemm_f64.hpp"
+#include "pobtaf_dag.hpp"
 #include "solve_f64.hpp"
 #include "tile_f64.hpp"
+
+#include <cstdlib>
+#include <cstring>
 
 using namespace rocbta;
 
@@ -281,6 +285,87 @@ SiLayout si_layout(int64_t n, int64_t b, int64_t a) {
   return l;
 }
 
+struct AfLayout {
+  size_t tinv, counter, flags, linv, tipp, total;
+  int ntasks, tpb;
+};
+
+AfLayout af_layout(int64_t n, int64_t b, int64_t a) {
+  AfLayout l{};
+  const int64_t nt = (b + DT - 1) / DT, ha = a > 0 ? 1 : 0;
+  l.tpb = (int)(nt * (2 * nt + ha) - (nt * (nt - 1)) / 2 + ha);
+  l.ntasks = (int)(n * l.tpb + ha);
+  size_t off = 0;
+  auto take = [&](size_t bytes) {
+    size_t o = off;
+    off = align256(off + bytes);
+    return o;
+  };
+  l.tinv = take(TT * TT * sizeof(double));
+  l.counter = take(sizeof(int));
+  l.flags = take((size_t)l.ntasks * sizeof(int));
+  l.linv = take((size_t)n * nt * DT * DT * sizeof(double));
+  l.tipp = take((size_t)n * a * a * sizeof(double));
+  l.total = off;
+  return l;
+}
+
+__global__ void fill_i32_kernel(int* p, int64_t count, int v) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (; i < count; i += stride) p[i] = v;
+}
+
+int device_cus() {
+  static int cus[64] = {0};
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return 256;
+  if (cus[dev] == 0) {
+    int v = 0;
+    if (hipDeviceGetAttribute(&v, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || v <= 0)
+      v = 256;
+    cus[dev] = v;
+  }
+  return cus[dev];
+}
+
+// ROCBTA_POBTAF_ALGO=loop forces the launch-per-tile path (used by tests to
+// cover both paths); default: the persistent tile DAG whenever it applies.
+bool pobtaf_use_dag(int64_t a, int factorize_last_block) {
+  const char* e = getenv("ROCBTA_POBTAF_ALGO");
+  if (e && strcmp(e, "loop") == 0) return false;
+  return factorize_last_block && a <= DT;
+}
+
+int pobtaf_dag(int64_t n, int64_t b, int64_t a, double* D, double* L, double* R, double* T,
+               int* info, char* ws, const AfLayout& lay, hipStream_t s) {
+  PobtafDag P{};
+  P.D = D;
+  P.L = L;
+  P.R = R;
+  P.T = T;
+  P.n = (int)n;
+  P.b = (int)b;
+  P.a = (int)a;
+  P.nt = (int)((b + DT - 1) / DT);
+  P.ha = a > 0 ? 1 : 0;
+  P.tpb = lay.tpb;
+  P.ntasks = lay.ntasks;
+  P.counter = (int*)(ws + lay.counter);
+  P.flags = (int*)(ws + lay.flags);
+  P.info = info;
+  P.linv = (double*)(ws + lay.linv);
+  P.tipp = (double*)(ws + lay.tipp);
+  P.spin_limit = 1ll << 22;
+  const int64_t nflag = (int64_t)((lay.linv - lay.counter) / sizeof(int));
+  fill_i32_kernel<<<(int)std::min<int64_t>(1024, (nflag + 255) / 256), 256, 0, s>>>(P.counter, nflag, 0);
+  RB_CHECK(hipGetLastError());
+  const int grid = std::max(1, std::min(P.ntasks, 2 * device_cus()));
+  pobtaf_dag_kernel<<<grid, DAG_THREADS, 0, s>>>(P);
+  RB_CHECK(hipGetLastError());
+  return ROCBTA_OK;
+}
+
 bool bad_shape(int64_t n, int64_t b, int64_t a) {
   return n < 1 || b < 1 || a < 0 || b > (1 << 20) || a > (1 << 20) || n > (1 << 24);
 }
@@ -297,7 +382,7 @@ size_t rocbta_workspace_bytes(int op, int64_t n, int64_t b, int64_t a, int64_t r
   if (bad_shape(n, b, a)) return 0;
   switch (op) {
     case ROCBTA_OP_POBTAF:
-      return align256(TT * TT * sizeof(double));
+      return af_layout(n, b, a).total;
     case ROCBTA_OP_POBTASI:
       return si_layout(n, b, a).total;
     case ROCBTA_OP_POBTAS:
@@ -315,11 +400,13 @@ int rocbta_dpobtaf(int64_t n, int64_t b, int64_t a, double* D, double* L, double
   if (!workspace || workspace_bytes < rocbta_workspace_bytes(ROCBTA_OP_POBTAF, n, b, a, 0))
     return ROCBTA_ERR_WORKSPACE;
   hipStream_t s = (hipStream_t)stream;
-  double* tinv = (double*)workspace;
-  if (info) {
-    set_int_kernel<<<1, 64, 0, s>>>(info, 0);
-    RB_CHECK(hipGetLastError());
-  }
+  const AfLayout lay = af_layout(n, b, a);
+  double* tinv = (double*)((char*)workspace + lay.tinv);
+  if (!info) return ROCBTA_ERR_ARG;
+  set_int_kernel<<<1, 64, 0, s>>>(info, 0);
+  RB_CHECK(hipGetLastError());
+  if (pobtaf_use_dag(a, factorize_last_block))
+    return pobtaf_dag(n, b, a, D, L, R, T, info, (char*)workspace, lay, s);
   const int bi = (int)b, ai = (int)a;
   const int64_t bb = b * b, ab = a * b;
   for (int64_t i = 0; i + 1 < n; ++i) {
@@ -568,6 +655,68 @@ int rocbta_dpobtas(int64_t n, int64_t b, int64_t a, const double* D, const doubl
     }
   }
   return ROCBTA_OK;
+}
+
+int64_t rocbta_dag_selfcheck(int64_t n, int64_t b, int64_t a) {
+  if (bad_shape(n, b, a) || a > DT) return -1;
+  const AfLayout lay = af_layout(n, b, a);
+  PobtafDag P{};
+  double* const base = reinterpret_cast<double*>(uintptr_t(1) << 40);
+  P.D = base;
+  P.L = base + ((int64_t)1 << 36);
+  P.R = base + ((int64_t)2 << 36);
+  P.T = base + ((int64_t)3 << 36);
+  P.n = (int)n;
+  P.b = (int)b;
+  P.a = (int)a;
+  P.nt = (int)((b + DT - 1) / DT);
+  P.ha = a > 0 ? 1 : 0;
+  P.tpb = lay.tpb;
+  P.ntasks = lay.ntasks;
+  int64_t bad = 0;
+  std::vector<char> seen((size_t)P.ntasks, 0);
+  auto in_range = [&](const DagTile& t, const double* arr, int64_t count) {
+    const int64_t off = t.p - arr;
+    if (t.rows <= 0 || t.cols <= 0 || t.rows > DT || t.cols > DT) return false;
+    return off >= 0 && off + (int64_t)(t.rows - 1) * t.ld + (t.cols - 1) < count;
+  };
+  auto tile_ok = [&](const DagTile& t) {
+    return in_range(t, P.D, n * b * b) || in_range(t, P.L, (n - 1) * b * b) ||
+           in_range(t, P.R, n * a * b);
+  };
+  for (int t = 0; t < P.ntasks; ++t) {
+    int kind, i, r, c;
+    dag_decode(P, t, kind, i, r, c);
+    int back = -1;
+    if (kind == K_DTILE) back = (i < P.n && c <= r && r < P.nt) ? t_offd(P, i, r, c) : -1;
+    if (kind == K_LOW) back = (i < P.n && r < P.nt && c < P.nt) ? t_low(P, i, r, c) : -1;
+    if (kind == K_ARR) back = (P.ha && i < P.n && c < P.nt) ? t_arr(P, i, c) : -1;
+    if (kind == K_TIPP) back = (P.ha && i < P.n) ? t_tipp(P, i) : -1;
+    if (kind == K_TIP) back = P.n * P.tpb;
+    if (back != t || seen[t]) ++bad;
+    seen[t] = 1;
+    if (kind == K_TIP) {
+      for (int ii = 0; ii < P.n; ++ii)
+        if (t_tipp(P, ii) >= t) ++bad;
+      continue;
+    }
+    if (kind == K_LOW && i == P.n - 1) continue;
+    Contrib cb;
+    for (int j = 0; dag_contrib(P, kind, i, r, c, j, cb); ++j) {
+      if (cb.fa < 0 || cb.fa >= t || cb.fb >= t) ++bad;
+      if (!tile_ok(cb.A) || !tile_ok(cb.B)) ++bad;
+      if (j > 2 * P.nt) {
+        ++bad;
+        break;
+      }
+    }
+    if ((kind == K_DTILE && r > c) || kind == K_LOW || kind == K_ARR)
+      if (t_diag(P, i, c) >= t) ++bad;
+    if (kind == K_DTILE && !tile_ok(tile_D(P, i, r, c))) ++bad;
+    if (kind == K_LOW && !tile_ok(tile_L(P, i, r, c))) ++bad;
+    if (kind == K_ARR && !tile_ok(tile_R(P, i, c))) ++bad;
+  }
+  return bad;
 }
 
 int rocbta_dlogdet(int64_t n, int64_t b, int64_t a, const double* D, const double* T,

@@ -4,50 +4,65 @@
 
 namespace rocbta {
 
-constexpr int SV_COLS = 64;  // right-hand-side columns per workgroup
+constexpr int SV_COLS = 16;  // right-hand-side columns per workgroup (4 per wave)
 constexpr int SV_THREADS = 256;
+
+__device__ __forceinline__ double sv_readlane(double v, int lane) {
+  const uint64_t u = __builtin_bit_cast(uint64_t, v);
+  const uint32_t lo = __builtin_amdgcn_readlane((uint32_t)u, lane);
+  const uint32_t hi = __builtin_amdgcn_readlane((uint32_t)(u >> 32), lane);
+  return __builtin_bit_cast(double, ((uint64_t)hi << 32) | lo);
+}
 
 // Solve op(L) X = B in place for one lower-triangular tile L (m x m, m <= 64)
 // and the m x r block B (row stride ldb); op = identity (trans = 0) or
-// transpose (trans = 1).  Workgroup x handles columns [64x, 64x + 64).
-// Four lanes per column split the substitution sums (k = q mod 4) and combine
-// them with lane shuffles; each lane only re-reads solution entries it wrote.
+// transpose (trans = 1).  Lane i of every wave owns row i; wave w of
+// workgroup x owns columns 16x + 4w .. 16x + 4w + 3.  Each substitution step
+// broadcasts the solved row by readlane, so the 64 sequential steps need no
+// workgroup barrier.
 __global__ void __launch_bounds__(SV_THREADS)
     trsv_tile_kernel(const double* Lt, int64_t ld, int m, double* B, int64_t ldb, int r,
                      int trans) {
   __shared__ double S[64][65];
-  __shared__ double Xs[64][SV_COLS + 1];
   const int tid = threadIdx.x;
   for (int idx = tid; idx < 64 * 64; idx += SV_THREADS) {
     int i = idx / 64, k = idx % 64;
     S[i][k] = (i < m && k <= i) ? Lt[(int64_t)i * ld + k] : 0.0;
   }
   __syncthreads();
-  const int c = tid >> 2, q = tid & 3;
-  const int col = blockIdx.x * SV_COLS + c;
-  const bool active = col < r;
+  const int lane = tid & 63, wave = tid >> 6;
+  const int c0 = blockIdx.x * SV_COLS + wave * 4;
+  if (c0 >= r) return;  // whole wave idle (no later barrier)
+  const bool row_ok = lane < m;
+  const double invd = row_ok ? 1.0 / S[lane][lane] : 0.0;
+  double x[4];
+#pragma unroll
+  for (int q = 0; q < 4; ++q)
+    x[q] = (row_ok && c0 + q < r) ? B[(int64_t)lane * ldb + c0 + q] : 0.0;
   if (!trans) {
-    for (int i = 0; i < m; ++i) {
-      double s = 0.0;
-      if (active)
-        for (int k = q; k < i; k += 4) s += S[i][k] * Xs[k][c];
-      s += __shfl_xor(s, 1, 4);
-      s += __shfl_xor(s, 2, 4);
-      if (active && (i & 3) == q) Xs[i][c] = (B[(int64_t)i * ldb + col] - s) / S[i][i];
+    for (int j = 0; j < m; ++j) {
+      const double lij = (lane > j && row_ok) ? S[lane][j] : 0.0;
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        if (lane == j) x[q] *= invd;
+        const double xj = sv_readlane(x[q], j);
+        x[q] -= lij * xj;
+      }
     }
   } else {
-    for (int i = m - 1; i >= 0; --i) {
-      double s = 0.0;
-      if (active)
-        for (int k = i + 1 + ((q - i - 1) & 3); k < m; k += 4) s += S[k][i] * Xs[k][c];
-      s += __shfl_xor(s, 1, 4);
-      s += __shfl_xor(s, 2, 4);
-      if (active && (i & 3) == q) Xs[i][c] = (B[(int64_t)i * ldb + col] - s) / S[i][i];
+    for (int j = m - 1; j >= 0; --j) {
+      const double lji = (lane < j) ? S[j][lane] : 0.0;
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        if (lane == j) x[q] *= invd;
+        const double xj = sv_readlane(x[q], j);
+        x[q] -= lji * xj;
+      }
     }
   }
-  // each lane writes back the rows it owns
-  if (active)
-    for (int i = q; i < m; i += 4) B[(int64_t)i * ldb + col] = Xs[i][c];
+#pragma unroll
+  for (int q = 0; q < 4; ++q)
+    if (row_ok && c0 + q < r) B[(int64_t)lane * ldb + c0 + q] = x[q];
 }
 
 // out = 2 * (sum_i sum_j log D[i][j][j] + sum_j log T[j][j]); deterministic

@@ -6,8 +6,11 @@
 //           tile column j records info = row_offset + j + 1 (LAPACK style).
 //   trtri : X = L^{-1} for a lower-triangular L (strict upper zero-filled).
 //
-// Tiles are described by a batch of (pointer, ld) computed from two batch
-// strides, so one launch can factor or invert many independent tiles.
+// Both are blocked by 16 columns.  The sequential part of a 16-column step
+// (pivot, scaling, rank-1 updates inside the panel) runs in ONE wave with the
+// panel rows spread over the 64 lanes and pivots broadcast by readlane, so it
+// needs no workgroup barrier; the trailing updates of the tile are 16x16x4
+// FP64 MFMA products spread over the four waves.
 #pragma once
 #include "common.hpp"
 
@@ -16,6 +19,7 @@ namespace rocbta {
 constexpr int TT = 64;  // maximum tile order
 constexpr int TT_THREADS = 256;
 constexpr int TT_LD = TT + 1;
+constexpr int PW = 16;  // panel width
 
 struct TileBatch {
   double* A;         // tile (0,0) of batch entry 0
@@ -34,55 +38,169 @@ struct TileBatch {
   int zcols;
 };
 
-// Cholesky of S (m x m, lower part valid) in place; all threads participate.
-__device__ void lds_potrf(double (*S)[TT_LD], int m, int* info, int64_t row0) {
-  const int tid = threadIdx.x;
-  for (int j = 0; j < m; ++j) {
-    if (tid == 0) {
-      double d = S[j][j];
-      if (!(d > 0.0)) {  // also catches NaN
-        record_info(info, (int)(row0 + j + 1));
-        d = __builtin_nan("");
+__device__ __forceinline__ double readlane_f64(double v, int lane) {
+  const uint64_t u = __builtin_bit_cast(uint64_t, v);
+  const uint32_t lo = __builtin_amdgcn_readlane((uint32_t)u, lane);
+  const uint32_t hi = __builtin_amdgcn_readlane((uint32_t)(u >> 32), lane);
+  return __builtin_bit_cast(double, ((uint64_t)hi << 32) | lo);
+}
+
+// Unblocked Cholesky of the panel S[p:m, p:p+w] (w <= 16, m - p <= 64) by
+// the calling wave; lane l owns row p + l.  Lower entries are written back.
+__device__ __forceinline__ void wave_panel_potrf(double (*S)[TT_LD], int p, int m, int w,
+                                                 int* info, int64_t row0) {
+  const int lane = threadIdx.x & 63;
+  const int i = p + lane;
+  const bool act = i < m;
+  double r[PW];
+#pragma unroll
+  for (int k = 0; k < PW; ++k) r[k] = (act && k < w) ? S[i][p + k] : 0.0;
+#pragma unroll
+  for (int j = 0; j < PW; ++j) {
+    if (j < w) {
+      double piv = readlane_f64(r[j], j);
+      if (!(piv > 0.0)) {
+        if (lane == 0) record_info(info, (int)(row0 + p + j + 1));
+        piv = __builtin_nan("");
       }
-      S[j][j] = sqrt(d);
+      const double d = sqrt(piv);
+      const double inv = 1.0 / d;
+      r[j] = (lane == j) ? d : (lane > j ? r[j] * inv : r[j]);
+#pragma unroll
+      for (int k = j + 1; k < PW; ++k) {
+        if (k < w) {
+          const double lkj = readlane_f64(r[j], k);
+          r[k] -= r[j] * lkj;
+        }
+      }
     }
-    __syncthreads();
-    const double inv = 1.0 / S[j][j];
-    if (tid > j && tid < m) S[tid][j] *= inv;
-    __syncthreads();
-    // trailing update of the lower triangle: columns k in (j, m)
-    const int k = tid & 63, g = tid >> 6;
-    if (k > j && k < m) {
-      const double lkj = S[k][j];
-      for (int i = k + g; i < m; i += 4) S[i][k] -= S[i][j] * lkj;
+  }
+#pragma unroll
+  for (int k = 0; k < PW; ++k)
+    if (act && k < w && p + k <= i) S[i][p + k] = r[k];
+}
+
+// S[q:m, q:m] (lower part) -= P P^T with P = S[q:m, p:p+w], on MFMA, all
+// waves.  Output 16x16 blocks are dealt round-robin to the waves.
+__device__ __forceinline__ void lds_syrk_update(double (*S)[TT_LD], int p, int w, int q, int m) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int nb = (m - q + 15) / 16;
+  const int li = lane & 15, lk = lane >> 4;
+  int t = 0;
+  for (int bi = 0; bi < nb; ++bi)
+    for (int bj = 0; bj <= bi; ++bj, ++t) {
+      if ((t & 3) != wave) continue;
+      f64x4 acc = {0.0, 0.0, 0.0, 0.0};
+      const int ra = q + bi * 16 + li, rb = q + bj * 16 + li;
+#pragma unroll
+      for (int kk = 0; kk < PW; kk += 4) {
+        const int k = kk + lk;
+        const double av = (k < w && ra < m) ? S[ra][p + k] : 0.0;
+        const double bv = (k < w && rb < m) ? S[rb][p + k] : 0.0;
+        acc = mfma_f64(av, bv, acc);
+      }
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int row = q + bi * 16 + acc_row_f64(lane, r);
+        const int col = q + bj * 16 + acc_col(lane);
+        if (row < m && col <= row) S[row][col] -= acc[r];
+      }
     }
+}
+
+// Blocked Cholesky of S (m x m, lower part valid) in place.
+__device__ void lds_potrf(double (*S)[TT_LD], int m, int* info, int64_t row0) {
+  const int wave = threadIdx.x >> 6;
+  for (int p = 0; p < m; p += PW) {
+    const int w = min(PW, m - p);
+    if (wave == 0) wave_panel_potrf(S, p, m, w, info, row0);
     __syncthreads();
+    if (p + w < m) {
+      lds_syrk_update(S, p, w, p + w, m);
+      __syncthreads();
+    }
   }
 }
 
-// X = L^{-1}, L lower (m x m) in S; X written to Xs (lower, upper zero).
-// Four lanes per column j: lane q sums the k = q (mod 4) part of the
-// forward substitution and keeps the X[k][j] it produced; the partial sums
-// are combined with lane shuffles, so no cross-lane LDS traffic is needed.
-__device__ void lds_trtri(const double (*S)[TT_LD], double (*Xs)[TT_LD], int m) {
+// X = L^{-1} (m x m lower, in S) into Xs (lower; strict upper zero).
+// Step 1: each wave inverts 16x16 diagonal blocks by Gauss-Jordan sweeps,
+// lane = (row i, column group g of four columns).  Step 2: off-diagonal
+// blocks by increasing distance d from the diagonal on MFMA:
+//   X_IJ = -X_II * sum_{K=J}^{I-1} L_IK X_KJ.
+// Ws: 64 x 17 scratch; at distance d the block (I, I-d) of W sits in rows
+// 16I .. 16I+15.
+constexpr int WS_LD = 17;
+__device__ void lds_trtri(const double (*S)[TT_LD], double (*Xs)[TT_LD], double (*Ws)[WS_LD],
+                          int m) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int tid = threadIdx.x;
-  const int j = tid >> 2, q = tid & 3;
-  for (int i = 0; i < m; ++i) {
-    if (j < m && i >= j) {
-      double s = 0.0;
-      for (int k = j + ((q - j) & 3); k < i; k += 4) s += S[i][k] * Xs[k][j];
-      s += __shfl_xor(s, 1, 4);
-      s += __shfl_xor(s, 2, 4);
-      const double x = ((i == j ? 1.0 : 0.0) - s) / S[i][i];
-      if ((i & 3) == q) Xs[i][j] = x;
+  for (int idx = tid; idx < TT * TT; idx += TT_THREADS) Xs[idx / TT][idx % TT] = 0.0;
+  __syncthreads();
+  const int nb = (m + 15) / 16;
+  for (int blk = wave; blk < nb; blk += 4) {
+    const int o = blk * 16, w = min(16, m - o);
+    const int i = lane & 15, g = lane >> 4;
+    double x[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) x[q] = (i == g * 4 + q) ? 1.0 : 0.0;
+    for (int j = 0; j < w; ++j) {
+      if (i == j) {
+        const double inv = 1.0 / S[o + j][o + j];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) x[q] *= inv;
+      }
+      const double lij = (i > j && i < w) ? S[o + i][o + j] : 0.0;
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const double xj = __shfl(x[q], g * 16 + j, 64);
+        x[q] -= lij * xj;
+      }
+    }
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const int c = g * 4 + q;
+      if (i < w && c < w && c <= i) Xs[o + i][o + c] = x[q];
     }
   }
   __syncthreads();
-  for (int idx = tid; idx < TT * TT; idx += TT_THREADS) {
-    int r = idx / TT, c = idx % TT;
-    if (r < m && c < m && c > r) Xs[r][c] = 0.0;
+  const int li = lane & 15, lk = lane >> 4;
+  for (int d = 1; d < nb; ++d) {
+    // W_J = sum_{K=J}^{J+d-1} L_{J+d,K} X_{K,J}  (into Ws block row J+d, col J)
+    for (int J = wave; J + d < nb; J += 4) {
+      const int I = J + d;
+      f64x4 acc = {0.0, 0.0, 0.0, 0.0};
+      for (int K = J; K < I; ++K)
+#pragma unroll
+        for (int kk = 0; kk < 16; kk += 4) {
+          const int k = K * 16 + kk + lk;
+          const int ra = I * 16 + li, cb = J * 16 + li;
+          const double av = (ra < m && k < m) ? S[ra][k] : 0.0;
+          const double bv = (k < m && cb < m) ? Xs[k][cb] : 0.0;
+          acc = mfma_f64(av, bv, acc);
+        }
+#pragma unroll
+      for (int r = 0; r < 4; ++r) Ws[I * 16 + acc_row_f64(lane, r)][acc_col(lane)] = acc[r];
+    }
+    __syncthreads();
+    for (int J = wave; J + d < nb; J += 4) {
+      const int I = J + d;
+      f64x4 acc = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+      for (int kk = 0; kk < 16; kk += 4) {
+        const int k = I * 16 + kk + lk;
+        const int ra = I * 16 + li;
+        const double av = (ra < m && k < m) ? Xs[ra][k] : 0.0;
+        const double bv = (k < m) ? Ws[k][li] : 0.0;
+        acc = mfma_f64(av, bv, acc);
+      }
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int row = I * 16 + acc_row_f64(lane, r), col = J * 16 + acc_col(lane);
+        if (row < m && col < m) Xs[row][col] = -acc[r];
+      }
+    }
+    __syncthreads();
   }
-  __syncthreads();
 }
 
 // mode 0: potrf (+ inverse into X if X != nullptr); mode 1: trtri only.
@@ -90,6 +208,7 @@ template <int MODE>
 __global__ void __launch_bounds__(TT_THREADS) tile_factor_kernel(TileBatch tb) {
   __shared__ double S[TT][TT_LD];
   __shared__ double Xs[TT][TT_LD];
+  __shared__ double Ws[TT][WS_LD];
   const int bz = blockIdx.x;
   const int b1 = bz % tb.batch1, b2 = bz / tb.batch1;
   double* A = tb.A + b1 * tb.sA1 + b2 * tb.sA2;
@@ -115,7 +234,7 @@ __global__ void __launch_bounds__(TT_THREADS) tile_factor_kernel(TileBatch tb) {
     }
   }
   if (tb.X != nullptr) {
-    lds_trtri(S, Xs, m);
+    lds_trtri(S, Xs, Ws, m);
     double* X = tb.X + b1 * tb.sX1 + b2 * tb.sX2;
     for (int idx = tid; idx < m * m; idx += TT_THREADS) {
       int r = idx / m, c = idx % m;

@@ -56,3 +56,15 @@ def test_argument_errors_return_before_launch(lib):
     fake = ctypes.c_void_p(16)
     assert lib.rocbta_dpobtas(2, 4, 1, fake, fake, fake, fake, fake, 1, 1, b"X", 0, None, 0, None) == _lib.ROCBTA_ERR_ARG
     assert lib.rocbta_dpobtaf(2, 4, 1, fake, fake, fake, fake, 1, None, None, 0, None) == _lib.ROCBTA_ERR_WORKSPACE
+
+
+@pytest.mark.parametrize("a", [0, 1, 2, 16, 31, 32, 63, 64])
+def test_pobtaf_dag_schedule_selfcheck(lib, a):
+    """The persistent pobtaf kernel's task decoding and dependency lists,
+    executed on the host: every wait targets an earlier task, every index
+    decodes back to itself, every tile lies inside its array."""
+    for n in (1, 2, 3, 7):
+        for b in (1, 2, 3, 17, 63, 64, 65, 100, 128, 130, 192):
+            assert lib.rocbta_dag_selfcheck(n, b, a) == 0, (n, b, a)
+    assert lib.rocbta_dag_selfcheck(256, 1024, 32) == 0
+    assert lib.rocbta_dag_selfcheck(4, 8, 65) == -1  # arrow wider than one tile: not this kernel

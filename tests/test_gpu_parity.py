@@ -26,6 +26,17 @@ def algs(gpu_device):
     return A
 
 
+@pytest.fixture(params=["dag", "loop"])
+def pobtaf_algo(request, monkeypatch):
+    """Run the test with the persistent tile-DAG pobtaf and with the
+    launch-per-tile pobtaf (ROCBTA_POBTAF_ALGO=loop)."""
+    if request.param == "loop":
+        monkeypatch.setenv("ROCBTA_POBTAF_ALGO", "loop")
+    else:
+        monkeypatch.delenv("ROCBTA_POBTAF_ALGO", raising=False)
+    return request.param
+
+
 def oracle_all(D, L, R, T, B):
     D, L, R, T, B = (x.copy() for x in (D, L, R, T, B))
     serinv_np.pobtaf(D, L, R, T)
@@ -72,7 +83,7 @@ REF_GRID = [(b, a, n, r) for b in (2, 3) for a in (2, 3) for n in (1, 2, 3, 4) f
 
 
 @pytest.mark.parametrize("b,a,n,r", REF_GRID)
-def test_reference_grid(algs, gpu_device, b, a, n, r):
+def test_reference_grid(algs, gpu_device, pobtaf_algo, b, a, n, r):
     rng = np.random.default_rng(10_000 + 1000 * b + 100 * a + 10 * n + r)
     A = generators.dd_bta(b, a, n, rng)
     generators.symmetrize(A)
@@ -82,16 +93,17 @@ def test_reference_grid(algs, gpu_device, b, a, n, r):
 
 
 TILE_SHAPES = [(3, 64, 16, 2), (2, 65, 1, 1), (4, 100, 70, 3), (2, 128, 64, 5), (3, 130, 0, 2),
-               (1, 200, 33, 1), (5, 63, 63, 64), (2, 1, 1, 1), (6, 8, 130, 70)]
+               (1, 200, 33, 1), (5, 63, 63, 64), (2, 1, 1, 1), (6, 8, 130, 70), (3, 256, 32, 4),
+               (2, 192, 64, 1)]
 
 
 @pytest.mark.parametrize("n,b,a,r", TILE_SHAPES)
-def test_tile_boundaries(algs, gpu_device, n, b, a, r):
+def test_tile_boundaries(algs, gpu_device, pobtaf_algo, n, b, a, r):
     D, L, R, T, B = bta_spd(n, b, a, seed=500 + n + b + a, n_rhs=r)
     compare(gpu_all(algs, gpu_device, D, L, R, T, B), oracle_all(D, L, R, T, B), f"n{n}b{b}a{a}")
 
 
-def test_workload_n32_b64_a16(algs, gpu_device):
+def test_workload_n32_b64_a16(algs, gpu_device, pobtaf_algo):
     D, L, R, T, B = bta_spd(32, 64, 16, n_rhs=1)
     compare(gpu_all(algs, gpu_device, D, L, R, T, B), oracle_all(D, L, R, T, B), "n32b64a16")
 
@@ -105,8 +117,8 @@ def test_golden_vectors(algs, gpu_device, name):
     compare(got, ref, name)
 
 
-@pytest.mark.parametrize("n,b", [(1, 3), (4, 2), (3, 70), (5, 64)])
-def test_bt_routines(algs, gpu_device, n, b):
+@pytest.mark.parametrize("n,b", [(1, 3), (4, 2), (3, 70), (5, 64), (3, 200)])
+def test_bt_routines(algs, gpu_device, pobtaf_algo, n, b):
     D, L, _, _, B = bta_spd(n, b, 0, seed=77 + n, n_rhs=2)
     Do, Lo, Bo = D.copy(), L.copy(), B.copy()
     serinv_np.pobtf(Do, Lo)
