@@ -20,6 +20,7 @@
 
 #include <cstdlib>
 #include <cstring>
+#include <mutex>
 
 using namespace rocbta;
 
@@ -262,6 +263,31 @@ int block_trtri(const double* Ls, int64_t lds, int64_t sL, double* X, int64_t ld
 }
 
 // -------------------------------------------------------- workspace layout
+// Per-device side stream (default priority) with a growable event pool,
+// used to overlap independent precompute with the caller's stream.
+constexpr int SI_CHUNK = 16;
+struct SideStream {
+  hipStream_t s = nullptr;
+  std::vector<hipEvent_t> ev;
+};
+
+SideStream* side_stream(int nevents) {
+  static std::mutex mu;
+  static SideStream per_dev[64];
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return nullptr;
+  std::lock_guard<std::mutex> lock(mu);
+  SideStream& ss = per_dev[dev];
+  if (!ss.s && hipStreamCreateWithFlags(&ss.s, hipStreamNonBlocking) != hipSuccess) return nullptr;
+  while ((int)ss.ev.size() < nevents) {
+    hipEvent_t e;
+    if (hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess) return nullptr;
+    ss.ev.push_back(e);
+  }
+  return &ss;
+}
+
+
 struct SiLayout {
   size_t linv, g, gt, w, tinv, wt, total;
 };
@@ -275,10 +301,11 @@ SiLayout si_layout(int64_t n, int64_t b, int64_t a) {
     return o;
   };
   const int64_t ntb = (b + TT - 1) / TT, nta = (a + TT - 1) / TT;
-  l.linv = take((size_t)n * b * b);
+  const int64_t nch = std::min<int64_t>(n, SI_CHUNK);  // L_ii^{-1} of one chunk at a time
+  l.linv = take((size_t)nch * b * b);
   l.g = take((size_t)std::max<int64_t>(n - 1, 0) * b * b);
   l.gt = take((size_t)n * a * b);
-  l.w = take((size_t)n * ntb * TT * TT);
+  l.w = take((size_t)nch * ntb * TT * TT);
   l.tinv = take((size_t)a * a);
   l.wt = take((size_t)nta * TT * TT);
   l.total = off;
@@ -469,25 +496,52 @@ int rocbta_dpobtasi(int64_t n, int64_t b, int64_t a, double* D, double* L, doubl
   // when the caller supplies X_{n-1,n-1}, X_{t,n-1}, X_T (pobtasi.py:114).
   const int npre = (int)(invert_last_block ? n : n - 1);
 
-  // (1) L_ii^{-1} for every stored block (independent across i)
-  int rc = block_trtri(D, b, bb, Linv, b, bb, bi, npre, W, s);
-  if (rc) return rc;
-  // (2) G_i = L_{i+1,i} L_ii^{-1}, Gt_i = L_{t,i} L_ii^{-1}, S_i = L_ii^{-T} L_ii^{-1} -> D_i
-  {
-    GemmList pre;
-    const int ng = std::min<int>(npre, (int)n - 1);
-    if (ng > 0)
-      push_term(pre.add(G, b, bi, bi, 0.0, TRI_NONE, ng, bb),
-                term(L, b, false, Linv, b, false, bi, 1.0, TRI_NONE, TRI_LOWER, bb, bb));
-    if (a > 0)
-      push_term(pre.add(Gt, b, ai, bi, 0.0, TRI_NONE, npre, ab),
-                term(R, b, false, Linv, b, false, bi, 1.0, TRI_NONE, TRI_LOWER, ab, bb));
-    push_term(pre.add(D, b, bi, bi, 0.0, TRI_NONE, npre, bb),
-              term(Linv, b, true, Linv, b, false, bi, 1.0, TRI_UPPER, TRI_LOWER, bb, bb));
-    rc = run_gemms(pre, s);
+  // (1)+(2) Independent per-block precompute, in chunks of blocks from the
+  // last one down, on a side stream so that it overlaps the sequential
+  // recurrence on the caller's stream:
+  //   L_ii^{-1}; G_i = L_{i+1,i} L_ii^{-1}; Gt_i = L_{t,i} L_ii^{-1};
+  //   S_i = L_ii^{-T} L_ii^{-1} -> D_i.
+  const int nchunks = (npre + SI_CHUNK - 1) / SI_CHUNK;
+  SideStream* side = side_stream(nchunks + 1);
+  if (!side) return ROCBTA_ERR_HIP;
+  RB_CHECK(hipEventRecord(side->ev[0], s));
+  RB_CHECK(hipStreamWaitEvent(side->s, side->ev[0], 0));
+  for (int k = 0; k < nchunks; ++k) {
+    const int hi = npre - k * SI_CHUNK;             // exclusive
+    const int lo = std::max(0, hi - SI_CHUNK);      // inclusive
+    const int cnt = hi - lo;
+    int rc = block_trtri(D + lo * bb, b, bb, Linv, b, bb, bi, cnt, W, side->s);
     if (rc) return rc;
+    GemmList pre;
+    const int ng = std::min<int>(hi, (int)n - 1) - lo;
+    if (ng > 0)
+      push_term(pre.add(G + lo * bb, b, bi, bi, 0.0, TRI_NONE, ng, bb),
+                term(L + lo * bb, b, false, Linv, b, false, bi, 1.0, TRI_NONE, TRI_LOWER, bb, bb));
+    if (a > 0)
+      push_term(pre.add(Gt + lo * ab, b, ai, bi, 0.0, TRI_NONE, cnt, ab),
+                term(R + lo * ab, b, false, Linv, b, false, bi, 1.0, TRI_NONE, TRI_LOWER, ab, bb));
+    // S_i is symmetric: lower tiles computed, mirrored to the upper part
+    GemmProblem& ps = pre.add(D + lo * bb, b, bi, bi, 0.0, TRI_LOWER, cnt, bb);
+    ps.mirror = 1;
+    push_term(ps, term(Linv, b, true, Linv, b, false, bi, 1.0, TRI_UPPER, TRI_LOWER, bb, bb));
+    rc = run_gemms(pre, side->s);
+    if (rc) return rc;
+    RB_CHECK(hipEventRecord(side->ev[k + 1], side->s));
   }
+  auto chunk_of = [&](int64_t blk) { return (int)((npre - 1 - blk) / SI_CHUNK); };
+  int waited = -1;
+  auto need = [&](int64_t blk) -> int {
+    const int k = chunk_of(blk);
+    while (waited < k) {
+      ++waited;
+      RB_CHECK(hipStreamWaitEvent(s, side->ev[waited + 1], 0));
+    }
+    return ROCBTA_OK;
+  };
+  int rc = ROCBTA_OK;
   if (invert_last_block) {  // pobtasi.py:114-139
+    rc = need(n - 1);
+    if (rc) return rc;
     double* Rl = R + (n - 1) * ab;
     double* Dl = D + (n - 1) * bb;
     if (a > 0) {
@@ -504,14 +558,17 @@ int rocbta_dpobtasi(int64_t n, int64_t b, int64_t a, double* D, double* L, doubl
       rc = run_gemms(xr, s);
       if (rc) return rc;
       GemmList xd;
-      push_term(xd.add(Dl, b, bi, bi, 1.0),
-                term(Rl, b, true, Gt + (n - 1) * ab, b, false, ai, -1.0));
+      GemmProblem& pd = xd.add(Dl, b, bi, bi, 1.0, TRI_LOWER);
+      pd.mirror = 1;
+      push_term(pd, term(Rl, b, true, Gt + (n - 1) * ab, b, false, ai, -1.0));
       rc = run_gemms(xd, s);
       if (rc) return rc;
     }
   }
   // (3) backward recurrence (pobtasi.py:141-172) with the precomputed G's
   for (int64_t i = n - 2; i >= 0; --i) {
+    rc = need(i);
+    if (rc) return rc;
     GemmList off;
     GemmProblem& pl = off.add(L + i * bb, b, bi, bi, 0.0);
     push_term(pl, term(D + (i + 1) * bb, b, false, G + i * bb, b, false, bi, -1.0));
@@ -523,12 +580,20 @@ int rocbta_dpobtasi(int64_t n, int64_t b, int64_t a, double* D, double* L, doubl
     }
     rc = run_gemms(off, s);
     if (rc) return rc;
+    // X_ii = S_i - X_{i+1,i}^T G_i - X_{t,i}^T Gt_i is symmetric: lower
+    // tiles computed, mirrored to the upper part.
     GemmList dia;
-    GemmProblem& pd = dia.add(D + i * bb, b, bi, bi, 1.0);
+    GemmProblem& pd = dia.add(D + i * bb, b, bi, bi, 1.0, TRI_LOWER);
+    pd.mirror = 1;
     push_term(pd, term(L + i * bb, b, true, G + i * bb, b, false, bi, -1.0));
     if (a > 0) push_term(pd, term(R + i * ab, b, true, Gt + i * ab, b, false, ai, -1.0));
     rc = run_gemms(dia, s);
     if (rc) return rc;
+  }
+  // every chunk has been waited for by now (the recurrence reaches block 0)
+  while (waited + 1 < nchunks) {
+    ++waited;
+    RB_CHECK(hipStreamWaitEvent(s, side->ev[waited + 1], 0));
   }
   return ROCBTA_OK;
 }

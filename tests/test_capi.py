@@ -44,7 +44,8 @@ def test_workspace_sizes(lib):
     n, b, a = 256, 1024, 32
     assert lib.rocbta_workspace_bytes(_lib.OP_POBTAF, n, b, a, 0) >= 64 * 64 * 8
     si = lib.rocbta_workspace_bytes(_lib.OP_POBTASI, n, b, a, 0)
-    assert si >= (2 * n - 1) * b * b * 8  # L^{-1} blocks and G blocks
+    assert si >= (n - 1) * b * b * 8  # the G_i = L_{i+1,i} L_ii^{-1} blocks
+    assert si < (n + 64) * b * b * 8  # L_ii^{-1} is kept for one chunk of blocks only
     assert lib.rocbta_workspace_bytes(_lib.OP_POBTAF, 0, b, a, 0) == 0  # invalid shape
     assert lib.rocbta_workspace_bytes(99, n, b, a, 0) == 0
 
