@@ -50,6 +50,13 @@ struct GemmProblem {
   int mirror;  // with uplo == TRI_LOWER: also write C[col][row] = C[row][col]
   int tiles_m, tiles_n;
   int tile_begin;  // first global tile index of this problem
+  // Split-K: ksplit workgroups share one output tile, each over a contiguous
+  // part of the K stages; partial tiles go to `partial` (ksplit * 4096
+  // doubles per tile) and the last workgroup to finish a tile (per-tile
+  // counter, zero on entry and reset on exit) sums them in split order.
+  int ksplit;
+  double* partial;
+  int* counters;
   GemmTerm t[2];
 };
 
@@ -59,69 +66,69 @@ struct GemmGroup {
   GemmProblem p[ROCBTA_MAX_PROBLEMS];
 };
 
-// One pipeline stage: alpha*op(A) (64 x GT_K) and op(B) (GT_K x 64) of one
-// term, 8 + 8 doubles per thread.  Out-of-range elements are zero.
+// ------------------------------------------------------------------ staging
+// A pipeline stage holds a 64 x GT_K slice of alpha*op(A) and of op(B)^T
+// (rows m resp. n, columns k).  Each operand is read from global memory in
+// its natural orientation with 16-byte loads (two doubles per lane, whole
+// 256/512-byte row segments per wave) and kept in LDS in that orientation:
+//   K-contiguous source (A not transposed, B transposed): X[r][k], pitch PK
+//   R-contiguous source (A transposed,     B not transposed): X[k][r], pitch PR
+// An MFMA operand fetch (16 consecutive r at one k, four k per wave) then
+// reads LDS with a small stride in either layout.
+constexpr int PK = GT_K + 2;    // 34 doubles: rows of a K-contiguous slice
+constexpr int PR = GT_M + 2;    // 66 doubles: rows of an R-contiguous slice
+constexpr int SLICE = 64 * PK;  // doubles per operand slot (>= GT_K * PR)
+static_assert(GT_K * PR <= SLICE, "LDS slot too small");
+
+typedef double dbl2 __attribute__((ext_vector_type(2)));
+
 struct StageRegs {
-  double a[8];
-  double b[8];
+  dbl2 v[4];  // this thread's 8 elements of one operand slice
 };
 
-// Register <-> LDS mapping of a stage (k-major LDS: As[k][m], Bs[k][n]):
-//   A not transposed (row-major M x K): thread t: m = t >> 2, k = 8*(t&3) .. +7
-//   A transposed (stored K x M):        thread t: k = t >> 3, m = 8*(t&7) .. +7
-//   B not transposed (stored K x N):    thread t: k = t >> 3, n = 8*(t&7) .. +7
-//   B transposed (stored N x K):        thread t: n = t >> 2, k = 8*(t&3) .. +7
-__device__ __forceinline__ void gemm_load_stage(const GemmTerm& t, const double* A,
-                                                const double* B, int M, int N, int m0, int n0,
-                                                int k0, int kend, StageRegs& r) {
+// Global -> registers.  ld: row stride of the stored matrix.  For a
+// K-contiguous slice the stored rows are r (64 of them), columns k; for an
+// R-contiguous slice the stored rows are k (GT_K of them), columns r.
+// rows_ok / cols_ok bound the stored row / column index (exclusive).
+template <bool KCONT, bool FAST>
+__device__ __forceinline__ void slice_load(const double* base, int64_t ld, int row0, int col0,
+                                           int rows_ok, int cols_ok, double alpha,
+                                           StageRegs& r) {
   const int tid = threadIdx.x;
-  if (!t.transA) {
-    const int m = m0 + (tid >> 2), kq = k0 + (tid & 3) * 8;
-    const double* src = A + (int64_t)m * t.lda + kq;
+  const int rsub = KCONT ? (tid >> 4) : (tid >> 5);
+  const int c = KCONT ? (tid & 15) * 2 : (tid & 31) * 2;
+  const int rstep = KCONT ? 16 : 8;
 #pragma unroll
-    for (int j = 0; j < 8; ++j) r.a[j] = (m < M && kq + j < kend) ? t.alpha * src[j] : 0.0;
-  } else {
-    const int k = k0 + (tid >> 3), mq = m0 + (tid & 7) * 8;
-    const double* src = A + (int64_t)k * t.lda + mq;
-#pragma unroll
-    for (int j = 0; j < 8; ++j) r.a[j] = (k < kend && mq + j < M) ? t.alpha * src[j] : 0.0;
-  }
-  if (!t.transB) {
-    const int k = k0 + (tid >> 3), nq = n0 + (tid & 7) * 8;
-    const double* src = B + (int64_t)k * t.ldb + nq;
-#pragma unroll
-    for (int j = 0; j < 8; ++j) r.b[j] = (k < kend && nq + j < N) ? src[j] : 0.0;
-  } else {
-    const int n = n0 + (tid >> 2), kq = k0 + (tid & 3) * 8;
-    const double* src = B + (int64_t)n * t.ldb + kq;
-#pragma unroll
-    for (int j = 0; j < 8; ++j) r.b[j] = (n < N && kq + j < kend) ? src[j] : 0.0;
+  for (int q = 0; q < 4; ++q) {
+    const int row = row0 + rsub + rstep * q, col = col0 + c;
+    const double* src = base + (int64_t)row * ld + col;
+    dbl2 x;
+    if (FAST) {
+      x = *reinterpret_cast<const dbl2*>(__builtin_assume_aligned(src, 16));
+    } else {
+      x.x = (row < rows_ok && col < cols_ok) ? src[0] : 0.0;
+      x.y = (row < rows_ok && col + 1 < cols_ok) ? src[1] : 0.0;
+    }
+    r.v[q] = alpha * x;
   }
 }
 
-typedef double GemmLds[GT_K][GT_M + GT_LDS_PAD];
-
-__device__ __forceinline__ void gemm_store_stage(const GemmTerm& t, const StageRegs& r,
-                                                 GemmLds& As, GemmLds& Bs) {
+template <bool KCONT>
+__device__ __forceinline__ void slice_store(const StageRegs& r, double* X) {
   const int tid = threadIdx.x;
-  if (!t.transA) {
-    const int m = tid >> 2, kq = (tid & 3) * 8;
+  const int rsub = KCONT ? (tid >> 4) : (tid >> 5);
+  const int c = KCONT ? (tid & 15) * 2 : (tid & 31) * 2;
+  const int rstep = KCONT ? 16 : 8;
+  const int pitch = KCONT ? PK : PR;
 #pragma unroll
-    for (int j = 0; j < 8; ++j) As[kq + j][m] = r.a[j];
-  } else {
-    const int k = tid >> 3, mq = (tid & 7) * 8;
-#pragma unroll
-    for (int j = 0; j < 8; ++j) As[k][mq + j] = r.a[j];
-  }
-  if (!t.transB) {
-    const int k = tid >> 3, nq = (tid & 7) * 8;
-#pragma unroll
-    for (int j = 0; j < 8; ++j) Bs[k][nq + j] = r.b[j];
-  } else {
-    const int n = tid >> 2, kq = (tid & 3) * 8;
-#pragma unroll
-    for (int j = 0; j < 8; ++j) Bs[kq + j][n] = r.b[j];
-  }
+  for (int q = 0; q < 4; ++q)
+    *reinterpret_cast<dbl2*>(X + (rsub + rstep * q) * pitch + c) = r.v[q];
+}
+
+// Operand element (r, k) of a staged slice.
+template <bool KCONT>
+__device__ __forceinline__ double slice_at(const double* X, int r, int k) {
+  return KCONT ? X[r * PK + k] : X[k * PR + r];
 }
 
 // K range of a term restricted by the triangular shape of its operands.
@@ -136,7 +143,42 @@ __device__ __forceinline__ void gemm_k_range(const GemmTerm& t, int m0, int n0, 
   kbeg = (kbeg / GT_K) * GT_K;
 }
 
-__device__ __forceinline__ void gemm_mfma_stage(const GemmLds& As, const GemmLds& Bs,
+__device__ __forceinline__ bool aligned16(const double* p, int64_t ld) {
+  return ((reinterpret_cast<uintptr_t>(p) & 15) == 0) && ((ld & 1) == 0);
+}
+
+// One term's operands for one workgroup tile.  FAST: every stage is a full,
+// 16-byte aligned 64 x GT_K slice (no bounds checks, vector loads only).
+template <bool TA, bool TB, bool FAST>
+struct TermView {
+  const double* A;
+  const double* B;
+  int64_t lda, ldb;
+  int M, N, m0, n0, ke;
+  double alpha;
+
+  __device__ __forceinline__ void load(int k0, StageRegs& ra, StageRegs& rb) const {
+    // A: K-contiguous when not transposed (stored M x K), else R-contiguous (stored K x M)
+    if (!TA)
+      slice_load<true, FAST>(A, lda, m0, k0, M, ke, alpha, ra);
+    else
+      slice_load<false, FAST>(A, lda, k0, m0, ke, M, alpha, ra);
+    // B: op(B)(k, n); transposed => stored N x K (K-contiguous), else K x N
+    if (TB)
+      slice_load<true, FAST>(B, ldb, n0, k0, N, ke, 1.0, rb);
+    else
+      slice_load<false, FAST>(B, ldb, k0, n0, ke, N, 1.0, rb);
+  }
+  __device__ __forceinline__ void store(const StageRegs& ra, const StageRegs& rb, double* XA,
+                                        double* XB) const {
+    slice_store<!TA>(ra, XA);
+    slice_store<TB>(rb, XB);
+  }
+};
+
+// One LDS stage of MFMAs (wave quadrant 32x32 = 2x2 MFMA tiles).
+template <bool TA, bool TB>
+__device__ __forceinline__ void gemm_mfma_stage(const double* XA, const double* XB,
                                                 f64x4 (&acc)[2][2]) {
   const int lane = threadIdx.x & 63;
   const int wave = threadIdx.x >> 6;
@@ -144,10 +186,10 @@ __device__ __forceinline__ void gemm_mfma_stage(const GemmLds& As, const GemmLds
   const int li = lane & 15, lk = lane >> 4;
 #pragma unroll
   for (int kk = 0; kk < GT_K; kk += 4) {
-    const double a0 = As[kk + lk][wm + li];
-    const double a1 = As[kk + lk][wm + 16 + li];
-    const double b0 = Bs[kk + lk][wn + li];
-    const double b1 = Bs[kk + lk][wn + 16 + li];
+    const double a0 = slice_at<!TA>(XA, wm + li, kk + lk);
+    const double a1 = slice_at<!TA>(XA, wm + 16 + li, kk + lk);
+    const double b0 = slice_at<TB>(XB, wn + li, kk + lk);
+    const double b1 = slice_at<TB>(XB, wn + 16 + li, kk + lk);
     acc[0][0] = mfma_f64(a0, b0, acc[0][0]);
     acc[0][1] = mfma_f64(a0, b1, acc[0][1]);
     acc[1][0] = mfma_f64(a1, b0, acc[1][0]);
@@ -155,26 +197,73 @@ __device__ __forceinline__ void gemm_mfma_stage(const GemmLds& As, const GemmLds
   }
 }
 
-// Flat stage sequence over the (at most two) terms of a problem.
-struct StageSeq {
-  int nst[2];
-  int kb[2], ke[2];
-};
-
-__device__ __forceinline__ void stage_locate(const StageSeq& q, int s, int& term, int& k0) {
-  term = (s < q.nst[0]) ? 0 : 1;
-  const int ls = term ? s - q.nst[0] : s;
-  k0 = q.kb[term] + ls * GT_K;
+// acc += alpha * op(A) op(B) over the K range [kb, ke): LDS double buffer
+// (slots 0/1) and two register stages in flight (x/y), unrolled by two so
+// that every register set has a fixed role.
+template <bool TA, bool TB, bool FAST>
+__device__ __forceinline__ void gemm_term_pipeline(const TermView<TA, TB, FAST>& v, int kb,
+                                                   double* lds, f64x4 (&acc)[2][2]) {
+  const int ke = v.ke;
+  const int nst = kb < ke ? (ke - kb + GT_K - 1) / GT_K : 0;
+  if (nst == 0) return;
+  double* A0 = lds;
+  double* B0 = lds + SLICE;
+  double* A1 = lds + 2 * SLICE;
+  double* B1 = lds + 3 * SLICE;
+  StageRegs xa, xb, ya, yb;
+  __syncthreads();  // LDS free (previous term / previous use)
+  v.load(kb, xa, xb);
+  if (nst > 1) v.load(kb + GT_K, ya, yb);
+  v.store(xa, xb, A0, B0);
+  __syncthreads();
+  if (nst > 2) v.load(kb + 2 * GT_K, xa, xb);
+  for (int s = 0; s < nst; s += 2) {
+    // slot 0 holds stage s; y holds stage s+1; x holds stage s+2 (in flight)
+    gemm_mfma_stage<TA, TB>(A0, B0, acc);
+    if (s + 1 >= nst) break;
+    v.store(ya, yb, A1, B1);
+    __syncthreads();
+    if (s + 3 < nst) v.load(kb + (s + 3) * GT_K, ya, yb);
+    gemm_mfma_stage<TA, TB>(A1, B1, acc);
+    if (s + 2 >= nst) break;
+    v.store(xa, xb, A0, B0);
+    __syncthreads();
+    if (s + 4 < nst) v.load(kb + (s + 4) * GT_K, xa, xb);
+  }
 }
 
-__global__ void __launch_bounds__(GT_THREADS) gemm_group_kernel(GemmGroup g) {
-  __shared__ GemmLds As[2];
-  __shared__ GemmLds Bs[2];
+// Stages [sb, se) of the term (a stage = GT_K consecutive k from the term's
+// first k); the K range of those stages is clipped to the term's range.
+template <bool TA, bool TB>
+__device__ __forceinline__ void gemm_term(const GemmTerm& t, const double* A, const double* B,
+                                          int M, int N, int m0, int n0, int sb, int se,
+                                          double* lds, f64x4 (&acc)[2][2]) {
+  int kb, ke;
+  gemm_k_range(t, m0, n0, kb, ke);
+  ke = min(ke, kb + se * GT_K);
+  kb = kb + sb * GT_K;
+  if (kb >= ke) return;
+  const bool fast = aligned16(A, t.lda) && aligned16(B, t.ldb) && m0 + GT_M <= M &&
+                    n0 + GT_N <= N && ((ke - kb) % GT_K) == 0;
+  if (fast) {
+    TermView<TA, TB, true> v{A, B, t.lda, t.ldb, M, N, m0, n0, ke, t.alpha};
+    gemm_term_pipeline<TA, TB, true>(v, kb, lds, acc);
+  } else {
+    TermView<TA, TB, false> v{A, B, t.lda, t.ldb, M, N, m0, n0, ke, t.alpha};
+    gemm_term_pipeline<TA, TB, false>(v, kb, lds, acc);
+  }
+}
+
+__global__ void __launch_bounds__(GT_THREADS, 2) gemm_group_kernel(GemmGroup g) {
+  __shared__ __attribute__((aligned(16))) double lds[4 * SLICE];
   int tile = blockIdx.x;
   int pi = 0;
   while (pi + 1 < g.nprob && tile >= g.p[pi + 1].tile_begin) ++pi;
   const GemmProblem& P = g.p[pi];
-  int local = tile - P.tile_begin;
+  int local = (tile - P.tile_begin);
+  const int split = local % P.ksplit;
+  local /= P.ksplit;
+  const int tile_id = local;  // tile index within the problem (for split-K bookkeeping)
   const int per_batch = P.tiles_m * P.tiles_n;
   const int bidx = local / per_batch;
   local -= bidx * per_batch;
@@ -184,39 +273,10 @@ __global__ void __launch_bounds__(GT_THREADS) gemm_group_kernel(GemmGroup g) {
   if (P.uplo == TRI_LOWER && n0 > m0 + GT_M - 1) return;
   if (P.uplo == TRI_UPPER && m0 > n0 + GT_N - 1) return;
 
-  const double* Ab[2];
-  const double* Bb[2];
-  StageSeq q{};
-  for (int ti = 0; ti < 2; ++ti) {
-    q.nst[ti] = 0;
-    if (ti < P.nterms) {
-      const GemmTerm& t = P.t[ti];
-      Ab[ti] = t.A + b1 * t.sA1 + b2 * t.sA2;
-      Bb[ti] = t.B + b1 * t.sB1 + b2 * t.sB2;
-      gemm_k_range(t, m0, n0, q.kb[ti], q.ke[ti]);
-      q.nst[ti] = q.kb[ti] < q.ke[ti] ? (q.ke[ti] - q.kb[ti] + GT_K - 1) / GT_K : 0;
-    }
-  }
-  const int nst = q.nst[0] + q.nst[1];
-
   double* C = P.C + b1 * P.sC1 + b2 * P.sC2;
   const int lane = threadIdx.x & 63;
   const int wave = threadIdx.x >> 6;
   const int wm = (wave >> 1) * 32, wn = (wave & 1) * 32;
-
-  // Prefetch the C tile early (its latency then overlaps the main loop).
-  f64x4 cin[2][2];
-  const bool readc = P.beta != 0.0;
-#pragma unroll
-  for (int i = 0; i < 2; ++i)
-#pragma unroll
-    for (int j = 0; j < 2; ++j)
-#pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        const int row = m0 + wm + i * 16 + acc_row_f64(lane, r);
-        const int col = n0 + wn + j * 16 + acc_col(lane);
-        cin[i][j][r] = (readc && row < P.M && col < P.N) ? C[(int64_t)row * P.ldc + col] : 0.0;
-      }
 
   f64x4 acc[2][2];
 #pragma unroll
@@ -224,35 +284,68 @@ __global__ void __launch_bounds__(GT_THREADS) gemm_group_kernel(GemmGroup g) {
 #pragma unroll
     for (int j = 0; j < 2; ++j) acc[i][j] = f64x4{0.0, 0.0, 0.0, 0.0};
 
-  // Pipeline: LDS double buffer + two register stages in flight.
-  StageRegs r0, r1;
-  int term, k0;
-  if (nst > 0) {
-    stage_locate(q, 0, term, k0);
-    gemm_load_stage(P.t[term], Ab[term], Bb[term], P.M, P.N, m0, n0, k0, q.ke[term], r0);
-    gemm_store_stage(P.t[term], r0, As[0], Bs[0]);
+  // stage counts of the terms; this workgroup's share of the stage sequence
+  int nst[2] = {0, 0};
+  for (int ti = 0; ti < P.nterms; ++ti) {
+    int kb, ke;
+    gemm_k_range(P.t[ti], m0, n0, kb, ke);
+    nst[ti] = kb < ke ? (ke - kb + GT_K - 1) / GT_K : 0;
   }
-  if (nst > 1) {
-    stage_locate(q, 1, term, k0);
-    gemm_load_stage(P.t[term], Ab[term], Bb[term], P.M, P.N, m0, n0, k0, q.ke[term], r1);
+  const int total = nst[0] + nst[1];
+  const int g0 = (int)((int64_t)total * split / P.ksplit);
+  const int g1 = (int)((int64_t)total * (split + 1) / P.ksplit);
+  for (int ti = 0; ti < P.nterms; ++ti) {
+    const int off = ti ? nst[0] : 0;
+    const int sb = max(g0 - off, 0), se = min(g1 - off, nst[ti]);
+    if (sb >= se) continue;
+    const GemmTerm& t = P.t[ti];
+    const double* A = t.A + b1 * t.sA1 + b2 * t.sA2;
+    const double* B = t.B + b1 * t.sB1 + b2 * t.sB2;
+    if (!t.transA && !t.transB)
+      gemm_term<false, false>(t, A, B, P.M, P.N, m0, n0, sb, se, lds, acc);
+    else if (!t.transA && t.transB)
+      gemm_term<false, true>(t, A, B, P.M, P.N, m0, n0, sb, se, lds, acc);
+    else if (t.transA && !t.transB)
+      gemm_term<true, false>(t, A, B, P.M, P.N, m0, n0, sb, se, lds, acc);
+    else
+      gemm_term<true, true>(t, A, B, P.M, P.N, m0, n0, sb, se, lds, acc);
   }
-  if (nst > 2) {
-    stage_locate(q, 2, term, k0);
-    gemm_load_stage(P.t[term], Ab[term], Bb[term], P.M, P.N, m0, n0, k0, q.ke[term], r0);
-  }
-  __syncthreads();
-  for (int s = 0; s < nst; ++s) {
-    gemm_mfma_stage(As[s & 1], Bs[s & 1], acc);
-    if (s + 1 < nst) {
-      stage_locate(q, s + 1, term, k0);
-      gemm_store_stage(P.t[term], (s & 1) ? r0 : r1, As[(s + 1) & 1], Bs[(s + 1) & 1]);
-    }
-    if (s + 3 < nst) {
-      stage_locate(q, s + 3, term, k0);
-      gemm_load_stage(P.t[term], Ab[term], Bb[term], P.M, P.N, m0, n0, k0, q.ke[term],
-                      (s & 1) ? r0 : r1);
+
+  if (P.ksplit > 1) {
+    // Publish this split's partial tile; the last finisher sums all of them
+    // in split order (deterministic) and runs the epilogue.
+    const int tid = threadIdx.x;
+    double* mine = P.partial + ((int64_t)tile_id * P.ksplit + split) * (GT_M * GT_N);
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+      for (int j = 0; j < 2; ++j)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) mine[((i * 2 + j) * 4 + r) * GT_THREADS + tid] = acc[i][j][r];
+    __syncthreads();
+    __shared__ int s_last;
+    if (tid == 0) {
+      const int old = __hip_atomic_fetch_add(&P.counters[tile_id], 1, __ATOMIC_ACQ_REL,
+                                             __HIP_MEMORY_SCOPE_AGENT);
+      s_last = (old == P.ksplit - 1);
+      if (s_last) __hip_atomic_store(&P.counters[tile_id], 0, __ATOMIC_RELAXED,
+                                     __HIP_MEMORY_SCOPE_AGENT);
     }
     __syncthreads();
+    if (!s_last) return;
+    const double* base = P.partial + (int64_t)tile_id * P.ksplit * (GT_M * GT_N);
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+      for (int j = 0; j < 2; ++j)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int idx = ((i * 2 + j) * 4 + r) * GT_THREADS + tid;
+          double sum = 0.0;
+          for (int sp = 0; sp < P.ksplit; ++sp)
+            sum += (sp == split) ? acc[i][j][r] : base[(int64_t)sp * (GT_M * GT_N) + idx];
+          acc[i][j][r] = sum;
+        }
   }
 
 #pragma unroll
@@ -266,8 +359,9 @@ __global__ void __launch_bounds__(GT_THREADS) gemm_group_kernel(GemmGroup g) {
         if (row >= P.M || col >= P.N) continue;
         if (P.uplo == TRI_LOWER && col > row) continue;
         if (P.uplo == TRI_UPPER && col < row) continue;
-        const double v = acc[i][j][r] + P.beta * cin[i][j][r];
-        C[(int64_t)row * P.ldc + col] = v;
+        double* c = C + (int64_t)row * P.ldc + col;
+        const double v = (P.beta != 0.0) ? acc[i][j][r] + P.beta * (*c) : acc[i][j][r];
+        *c = v;
         if (P.mirror && col != row) C[(int64_t)col * P.ldc + row] = v;
       }
 }

@@ -58,6 +58,7 @@ struct GemmList {
     p.beta = beta;
     p.nterms = 0;
     p.uplo = uplo;
+    p.ksplit = 1;
     probs.push_back(p);
     return probs.back();
   }
@@ -105,7 +106,8 @@ int run_gemms(GemmList& list, hipStream_t s) {
     if (p.nterms == 0 && p.beta == 1.0) continue;
     p.tiles_m = cdiv(p.M, GT_M);
     p.tiles_n = cdiv(p.N, GT_N);
-    const int64_t tiles = (int64_t)p.tiles_m * p.tiles_n * p.batch1 * p.batch2;
+    if (p.ksplit < 1 || !p.partial || !p.counters) p.ksplit = 1;
+    const int64_t tiles = (int64_t)p.tiles_m * p.tiles_n * p.batch1 * p.batch2 * p.ksplit;
     if (g.nprob == ROCBTA_MAX_PROBLEMS || (int64_t)g.total_tiles + tiles > (1ll << 30)) {
       int rc = flush();
       if (rc) return rc;
@@ -115,6 +117,12 @@ int run_gemms(GemmList& list, hipStream_t s) {
     g.total_tiles += (int)tiles;
   }
   return flush();
+}
+
+__global__ void fill_i32_kernel(int* p, int64_t count, int v) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (; i < count; i += stride) p[i] = v;
 }
 
 int fill(double* p, int64_t count, double v, hipStream_t s) {
@@ -289,7 +297,7 @@ SideStream* side_stream(int nevents) {
 
 
 struct SiLayout {
-  size_t linv, g, gt, w, tinv, wt, total;
+  size_t linv, g, gt, w, tinv, wt, part, cnt, total;
 };
 
 SiLayout si_layout(int64_t n, int64_t b, int64_t a) {
@@ -308,6 +316,8 @@ SiLayout si_layout(int64_t n, int64_t b, int64_t a) {
   l.w = take((size_t)nch * ntb * TT * TT);
   l.tinv = take((size_t)a * a);
   l.wt = take((size_t)nta * TT * TT);
+  l.part = take((size_t)ntb * ntb * 2 * TT * TT);  // split-K partial tiles (2 splits)
+  l.cnt = take((size_t)(ntb * ntb + 64) / 2 + 1);  // split-K tile counters (ints)
   l.total = off;
   return l;
 }
@@ -335,12 +345,6 @@ AfLayout af_layout(int64_t n, int64_t b, int64_t a) {
   l.tipp = take((size_t)n * a * a * sizeof(double));
   l.total = off;
   return l;
-}
-
-__global__ void fill_i32_kernel(int* p, int64_t count, int v) {
-  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  int64_t stride = (int64_t)gridDim.x * blockDim.x;
-  for (; i < count; i += stride) p[i] = v;
 }
 
 int device_cus() {
@@ -490,6 +494,16 @@ int rocbta_dpobtasi(int64_t n, int64_t b, int64_t a, double* D, double* L, doubl
   double* W = (double*)(ws + lay.w);
   double* Tinv = (double*)(ws + lay.tinv);
   double* Wt = (double*)(ws + lay.wt);
+  double* part = (double*)(ws + lay.part);
+  int* cnt = (int*)(ws + lay.cnt);
+  const int ntb = cdiv(b, TT);
+  // Split the K loop of the recurrence's b x b products in two when one
+  // product alone has too few tiles to give every CU two workgroups.
+  const int ksplit = (ntb * ntb < 2 * device_cus() && b >= 256) ? 2 : 1;
+  if (ksplit > 1) {
+    fill_i32_kernel<<<1, 256, 0, s>>>(cnt, ntb * ntb, 0);
+    RB_CHECK(hipGetLastError());
+  }
   const int bi = (int)b, ai = (int)a;
   const int64_t bb = b * b, ab = a * b;
   // Blocks whose factor is still stored: all of them, or all but the last
@@ -571,6 +585,9 @@ int rocbta_dpobtasi(int64_t n, int64_t b, int64_t a, double* D, double* L, doubl
     if (rc) return rc;
     GemmList off;
     GemmProblem& pl = off.add(L + i * bb, b, bi, bi, 0.0);
+    pl.ksplit = ksplit;
+    pl.partial = part;
+    pl.counters = cnt;
     push_term(pl, term(D + (i + 1) * bb, b, false, G + i * bb, b, false, bi, -1.0));
     if (a > 0) {
       push_term(pl, term(R + (i + 1) * ab, b, true, Gt + i * ab, b, false, ai, -1.0));
@@ -585,6 +602,9 @@ int rocbta_dpobtasi(int64_t n, int64_t b, int64_t a, double* D, double* L, doubl
     GemmList dia;
     GemmProblem& pd = dia.add(D + i * bb, b, bi, bi, 1.0, TRI_LOWER);
     pd.mirror = 1;
+    pd.ksplit = ksplit;
+    pd.partial = part;
+    pd.counters = cnt;
     push_term(pd, term(L + i * bb, b, true, G + i * bb, b, false, bi, -1.0));
     if (a > 0) push_term(pd, term(R + i * ab, b, true, Gt + i * ab, b, false, ai, -1.0));
     rc = run_gemms(dia, s);
