@@ -1,0 +1,561 @@
+// Persistent block Cholesky of an SPD BTA matrix (pobtaf) on gfx950:
+// one CHAIN workgroup walks the diagonal tiles, HELPER workgroups do the rest.
+//
+// The reference recurrence (src/serinv/algs/pobtaf.py:115-183) is evaluated on
+// 64x64 tiles.  Tiles are addressed in global tile coordinates: tile row /
+// column g = i * nt + c is tile c of block i (nt = ceil(b / 64)); the band of
+// tile column j (block jb) holds rows j .. (jb + 2) * nt - 1 (the rest of D_jb
+// and all of L_{jb+1,jb}), plus the arrow tile of R_jb.  L(r, k) denotes a
+// finished factor tile.
+//
+// Chain workgroup, step g (strictly sequential):
+//   D(g,g) <- PD(g) - L(g,g-1) L(g,g-1)^T       (L(g,g-1) kept in LDS)
+//   L(g,g) = chol(D(g,g)), Linv(g) = L(g,g)^{-1}  -> published
+//   L(g+1,g) = PX(g) Linv(g)^T                    -> published, kept in LDS
+// where PD(g) = D(g,g) minus the products of all columns k < g-1 and
+// PX(g) = X(g+1,g) minus the products of all columns k < g are prepared by
+// helpers.  Helper tasks:
+//   TRSM(r, j), r >= j + 2 : X(r,j) - sum_k L(r,k) L(j,k)^T, then * Linv(j)^T
+//   ARR(j)                 : the same for the arrow tile of column j
+//   PX(g), PD(g)           : the partial sums above (stored in place)
+//   TIPP(i)                : P_i = R_i R_i^T;  TIP: chol(T - P_0 - ... - P_{n-1})
+// Helper tasks are taken from an atomic queue organised in stages; stage h
+// holds the tasks that need chain outputs of steps <= h - 1 only, and inside
+// a stage every task only waits on earlier slots.  The chain step g needs
+// PD(g) (stage g-1) and PX(g) (stage g), so no wait can cycle.  Every wait is
+// bounded: an overrun sets info = -1 and makes all later waits return.
+//
+// Memory ordering: tiles are published with an agent-scope release before
+// the flag store; waiters poll relaxed and issue one agent-scope acquire.
+#pragma once
+#include "common.hpp"
+#include "tile_f64.hpp"
+
+namespace rocbta {
+
+constexpr int CT = 64;          // tile size
+constexpr int CH_THREADS = 256;
+constexpr int KP = CT + 2;      // pitch of K-contiguous LDS tiles (doubles)
+
+struct PobtafChain {
+  double* D;
+  double* L;
+  double* R;
+  double* T;
+  int n, b, a, nt, ha;
+  int NT;       // number of diagonal tiles (n * nt)
+  int slots;    // helper slots per stage
+  int nstages;  // NT + 2
+  int ntasks;   // nstages * slots
+  int* counter;  // helper queue head
+  int* claim;    // chain claim ticket
+  int* hflags;   // helper task flags
+  int* cflags;   // chain flags: [2g] Linv(g) ready, [2g+1] L(g+1,g) ready
+  int* info;
+  double* linv;  // NT * 64 * 64
+  double* tipp;  // n * a * a
+  long long spin_limit;
+};
+
+struct CTile {
+  double* p;
+  int64_t ld;
+  int rows, cols;
+};
+
+enum ChainKind { HK_NOOP = 0, HK_TRSM = 1, HK_PX = 2, HK_PD = 3, HK_ARR = 4, HK_TIPP = 5, HK_TIP = 6 };
+
+__host__ __device__ __forceinline__ int cmin(int x, int y) { return x < y ? x : y; }
+__host__ __device__ __forceinline__ int cmax(int x, int y) { return x > y ? x : y; }
+
+// exclusive end of the band rows of tile column j
+__host__ __device__ __forceinline__ int band_end(const PobtafChain& P, int j) {
+  const int jb = j / P.nt;
+  return (jb < P.n - 1) ? (jb + 2) * P.nt : (jb + 1) * P.nt;
+}
+// first contributing column for a row of block rb
+__host__ __device__ __forceinline__ int kmin_of(const PobtafChain& P, int rb) {
+  return cmax(0, (rb - 1) * P.nt);
+}
+
+__host__ __device__ __forceinline__ CTile tile_g(const PobtafChain& P, int r, int j) {
+  const int jb = j / P.nt, jc = j % P.nt, rb = r / P.nt, rc = r % P.nt;
+  const int64_t b = P.b;
+  double* base = (rb == jb) ? P.D : P.L;
+  return {base + (int64_t)jb * b * b + (int64_t)rc * CT * b + (int64_t)jc * CT, b,
+          cmin(CT, P.b - rc * CT), cmin(CT, P.b - jc * CT)};
+}
+// Strict-upper D tile (jc, rc) that mirrors the lower tile (r, j) of the same
+// block; the factor has zeros there.
+__host__ __device__ __forceinline__ CTile tile_mirror(const PobtafChain& P, int r, int j) {
+  const int jb = j / P.nt, jc = j % P.nt, rc = r % P.nt;
+  const int64_t b = P.b;
+  return {P.D + (int64_t)jb * b * b + (int64_t)jc * CT * b + (int64_t)rc * CT, b,
+          cmin(CT, P.b - jc * CT), cmin(CT, P.b - rc * CT)};
+}
+__host__ __device__ __forceinline__ CTile tile_arrow(const PobtafChain& P, int j) {
+  const int jb = j / P.nt, jc = j % P.nt;
+  const int64_t b = P.b;
+  return {P.R + (int64_t)jb * P.a * b + (int64_t)jc * CT, b, P.a, cmin(CT, P.b - jc * CT)};
+}
+
+// ---------------------------------------------------------- task indexing
+// slot layout of stage h (j = h - 1):
+//   0             TRSM(j+2, j)
+//   1             PX(h)
+//   2             PD(h+1)
+//   3 .. 3+2nt-1  TRSM(j+3+q, j)
+//   3+2nt         ARR(j)
+//   4+2nt         TIPP(h/nt - 1) when h is a multiple of nt
+// stage NT+1, slot 0: TIP
+__host__ __device__ __forceinline__ int t_trsm(const PobtafChain& P, int r, int j) {
+  const int h = j + 1;
+  return h * P.slots + ((r == j + 2) ? 0 : 3 + (r - (j + 3)));
+}
+__host__ __device__ __forceinline__ int t_px(const PobtafChain& P, int g) { return g * P.slots + 1; }
+__host__ __device__ __forceinline__ int t_pd(const PobtafChain& P, int g) {
+  return (g - 1) * P.slots + 2;
+}
+__host__ __device__ __forceinline__ int t_arr(const PobtafChain& P, int j) {
+  return (j + 1) * P.slots + 3 + 2 * P.nt;
+}
+__host__ __device__ __forceinline__ int t_tippc(const PobtafChain& P, int i) {
+  return ((i + 1) * P.nt) * P.slots + 4 + 2 * P.nt;
+}
+__host__ __device__ __forceinline__ int t_tipc(const PobtafChain& P) {
+  return (P.NT + 1) * P.slots;
+}
+
+// Decode a helper task: kind, and (r, j) = output tile (TRSM), g in j (PX/PD),
+// column j (ARR), block i in j (TIPP).
+__host__ __device__ __forceinline__ int chain_decode(const PobtafChain& P, int task, int& r,
+                                                     int& j) {
+  const int h = task / P.slots, slot = task % P.slots;
+  r = 0;
+  j = 0;
+  if (h == P.NT + 1) return (slot == 0 && P.ha) ? HK_TIP : HK_NOOP;
+  const int jj = h - 1;
+  if (slot == 0) {
+    if (jj >= 0 && jj + 2 < band_end(P, jj)) {
+      r = jj + 2;
+      j = jj;
+      return HK_TRSM;
+    }
+    return HK_NOOP;
+  }
+  if (slot == 1) {
+    if (h < P.NT && h + 1 < band_end(P, h)) {
+      j = h;
+      return HK_PX;
+    }
+    return HK_NOOP;
+  }
+  if (slot == 2) {
+    if (h + 1 < P.NT) {
+      j = h + 1;
+      return HK_PD;
+    }
+    return HK_NOOP;
+  }
+  if (slot < 3 + 2 * P.nt) {
+    const int rr = jj + 3 + (slot - 3);
+    if (jj >= 0 && rr < band_end(P, jj)) {
+      r = rr;
+      j = jj;
+      return HK_TRSM;
+    }
+    return HK_NOOP;
+  }
+  if (slot == 3 + 2 * P.nt) {
+    if (P.ha && jj >= 0) {
+      j = jj;
+      return HK_ARR;
+    }
+    return HK_NOOP;
+  }
+  if (slot == 4 + 2 * P.nt && P.ha && h >= P.nt && h % P.nt == 0) {
+    j = h / P.nt - 1;
+    return HK_TIPP;
+  }
+  return HK_NOOP;
+}
+
+// Readiness flag of the factor tile (r, k), r > k (arrow: r < 0).
+__host__ __device__ __forceinline__ int* factor_flag(const PobtafChain& P, int r, int k) {
+  if (r < 0) return P.hflags + t_arr(P, k);
+  if (r == k + 1) return P.cflags + 2 * k + 1;
+  return P.hflags + t_trsm(P, r, k);
+}
+
+// Contribution list of a helper task: columns k in [k0, k1) with operands
+// A = factor tile (ra, k) (ra < 0: arrow tile k) and B = factor tile (jb, k).
+struct ContribRange {
+  int ra, jb, k0, k1;
+};
+__host__ __device__ __forceinline__ ContribRange chain_contribs(const PobtafChain& P, int kind,
+                                                               int r, int j) {
+  switch (kind) {
+    case HK_TRSM:
+      return {r, j, kmin_of(P, r / P.nt), j};
+    case HK_ARR:
+      return {-1, j, kmin_of(P, j / P.nt), j};
+    case HK_PX:
+      return {j + 1, j, kmin_of(P, (j + 1) / P.nt), j};
+    case HK_PD:
+      return {j, j, kmin_of(P, j / P.nt), j - 1};
+    default:
+      return {0, 0, 0, 0};
+  }
+}
+
+// ---------------------------------------------------------------- flags
+__device__ __forceinline__ bool chain_poll(const PobtafChain& P, const int* f, long long& spins) {
+  while (__hip_atomic_load(f, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0) {
+    if (++spins > P.spin_limit) return false;
+    if ((spins & 255) == 0 &&
+        __hip_atomic_load(P.info, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < 0)
+      return false;
+    __builtin_amdgcn_s_sleep(1);
+  }
+  return true;
+}
+
+// Wait for flags fa (and fb if non-null); all threads return after an acquire.
+__device__ __forceinline__ void chain_wait(const PobtafChain& P, const int* fa, const int* fb) {
+  if (threadIdx.x == 0) {
+    long long spins = 0;
+    bool ok = chain_poll(P, fa, spins);
+    if (ok && fb) ok = chain_poll(P, fb, spins);
+    if (!ok) __hip_atomic_store(P.info, -1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+  }
+  __syncthreads();
+}
+
+__device__ __forceinline__ void chain_publish(int* f) {
+  __syncthreads();  // all waves' stores of the tile have completed
+  if (threadIdx.x == 0) __hip_atomic_store(f, 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// ----------------------------------------------------------- tile movement
+typedef double cdbl2 __attribute__((ext_vector_type(2)));
+
+// Thread t holds rows (t >> 5) + 8q (q = 0..7), columns 2*(t & 31) .. +1 of a
+// 64x64 tile: each wave load instruction covers two whole 512-byte rows.
+struct TileRegs16 {
+  cdbl2 v[8];
+};
+
+__device__ __forceinline__ void tile_load16(const CTile& t, TileRegs16& x) {
+  const int rs = threadIdx.x >> 5, c = (threadIdx.x & 31) * 2;
+  const bool fast = t.rows == CT && t.cols == CT &&
+                    ((reinterpret_cast<uintptr_t>(t.p) & 15) == 0) && ((t.ld & 1) == 0);
+  if (fast) {
+#pragma unroll
+    for (int q = 0; q < 8; ++q)
+      x.v[q] = *reinterpret_cast<const cdbl2*>(
+          __builtin_assume_aligned(t.p + (int64_t)(rs + 8 * q) * t.ld + c, 16));
+  } else {
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+      const int row = rs + 8 * q;
+      const double* src = t.p + (int64_t)row * t.ld + c;
+      x.v[q].x = (row < t.rows && c < t.cols) ? src[0] : 0.0;
+      x.v[q].y = (row < t.rows && c + 1 < t.cols) ? src[1] : 0.0;
+    }
+  }
+}
+
+// K-contiguous staging: X[row][k] with pitch KP (row = m or n index).
+__device__ __forceinline__ void tile_store16(const TileRegs16& x, double* X) {
+  const int rs = threadIdx.x >> 5, c = (threadIdx.x & 31) * 2;
+#pragma unroll
+  for (int q = 0; q < 8; ++q) *reinterpret_cast<cdbl2*>(X + (rs + 8 * q) * KP + c) = x.v[q];
+}
+
+// acc (+)= sgn * A B^T over k in [0, 64), A = XA[m][k], B = XB[n][k].
+__device__ __forceinline__ void mfma_abt(const double* XA, const double* XB, f64x4 (&acc)[2][2],
+                                         double sgn, bool lower_only) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int wm = (wave >> 1) * 32, wn = (wave & 1) * 32;
+  if (lower_only && wn > wm) return;
+  const int li = lane & 15, lk = lane >> 4;
+#pragma unroll 4
+  for (int kk = 0; kk < CT; kk += 4) {
+    const double a0 = sgn * XA[(wm + li) * KP + kk + lk];
+    const double a1 = sgn * XA[(wm + 16 + li) * KP + kk + lk];
+    const double b0 = XB[(wn + li) * KP + kk + lk];
+    const double b1 = XB[(wn + 16 + li) * KP + kk + lk];
+    acc[0][0] = mfma_f64(a0, b0, acc[0][0]);
+    acc[0][1] = mfma_f64(a0, b1, acc[0][1]);
+    acc[1][0] = mfma_f64(a1, b0, acc[1][0]);
+    acc[1][1] = mfma_f64(a1, b1, acc[1][1]);
+  }
+}
+
+__device__ __forceinline__ void cacc_zero(f64x4 (&acc)[2][2]) {
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int j = 0; j < 2; ++j) acc[i][j] = f64x4{0.0, 0.0, 0.0, 0.0};
+}
+
+__device__ __forceinline__ void cacc_load(const CTile& t, f64x4 (&acc)[2][2]) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int wm = (wave >> 1) * 32, wn = (wave & 1) * 32;
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int j = 0; j < 2; ++j)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int row = wm + i * 16 + acc_row_f64(lane, r), col = wn + j * 16 + acc_col(lane);
+        acc[i][j][r] = (row < t.rows && col < t.cols) ? t.p[(int64_t)row * t.ld + col] : 0.0;
+      }
+}
+
+__device__ __forceinline__ void cacc_store(const CTile& t, const f64x4 (&acc)[2][2]) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int wm = (wave >> 1) * 32, wn = (wave & 1) * 32;
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int j = 0; j < 2; ++j)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int row = wm + i * 16 + acc_row_f64(lane, r), col = wn + j * 16 + acc_col(lane);
+        if (row < t.rows && col < t.cols) t.p[(int64_t)row * t.ld + col] = acc[i][j][r];
+      }
+}
+
+// Stage the accumulator in LDS: X[row * pitch + col] = acc(row, col).
+__device__ __forceinline__ void cacc_to_lds(const f64x4 (&acc)[2][2], double* X, int pitch) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int wm = (wave >> 1) * 32, wn = (wave & 1) * 32;
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int j = 0; j < 2; ++j)
+#pragma unroll
+      for (int r = 0; r < 4; ++r)
+        X[(wm + i * 16 + acc_row_f64(lane, r)) * pitch + wn + j * 16 + acc_col(lane)] =
+            acc[i][j][r];
+}
+
+// Zero a whole tile (used for the strict-upper tiles of the D blocks).
+__device__ __forceinline__ void tile_zero(const CTile& t) {
+  for (int idx = threadIdx.x; idx < CT * CT; idx += CH_THREADS) {
+    const int row = idx >> 6, col = idx & 63;
+    if (row < t.rows && col < t.cols) t.p[(int64_t)row * t.ld + col] = 0.0;
+  }
+}
+
+// ------------------------------------------------------------ helper work
+// acc -= sum over the contribution range (A = factor tile (ra,k) or arrow
+// tile k, B = factor tile (jb,k)); operands prefetched one product ahead.
+__device__ void helper_accumulate(const PobtafChain& P, const ContribRange& cr, double sgn,
+                                  bool lower_only, double* XA, double* XB, f64x4 (&acc)[2][2]) {
+  if (cr.k0 >= cr.k1) return;
+  TileRegs16 ra, rb;
+  auto tileA = [&](int k) { return cr.ra < 0 ? tile_arrow(P, k) : tile_g(P, cr.ra, k); };
+  chain_wait(P, factor_flag(P, cr.ra, cr.k0), factor_flag(P, cr.jb, cr.k0));
+  tile_load16(tileA(cr.k0), ra);
+  tile_load16(tile_g(P, cr.jb, cr.k0), rb);
+  for (int k = cr.k0;; ++k) {
+    __syncthreads();  // previous product done with XA / XB
+    tile_store16(ra, XA);
+    tile_store16(rb, XB);
+    __syncthreads();
+    const bool more = k + 1 < cr.k1;
+    if (more) {
+      chain_wait(P, factor_flag(P, cr.ra, k + 1), factor_flag(P, cr.jb, k + 1));
+      tile_load16(tileA(k + 1), ra);
+      tile_load16(tile_g(P, cr.jb, k + 1), rb);
+    }
+    mfma_abt(XA, XB, acc, sgn, lower_only);
+    if (!more) break;
+  }
+  __syncthreads();
+}
+
+// out = acc * Linv(j)^T
+__device__ void helper_solve(const PobtafChain& P, int j, double* XA, double* XB,
+                             f64x4 (&acc)[2][2]) {
+  chain_wait(P, P.cflags + 2 * j, nullptr);
+  TileRegs16 rl;
+  tile_load16({P.linv + (int64_t)j * CT * CT, CT, CT, CT}, rl);
+  cacc_to_lds(acc, XA, KP);  // A[m][k] = acc(m, k)
+  tile_store16(rl, XB);      // B[n][k] = Linv(n, k)
+  __syncthreads();
+  cacc_zero(acc);
+  mfma_abt(XA, XB, acc, 1.0, false);
+}
+
+// ------------------------------------------------------------- chain work
+// LDS plan of the chain workgroup: S [64][65] (potrf, then L(g+1,g) staged
+// K-contiguous with pitch 65 for the next step's update), Xs [64][65]
+// (Linv(g)), Ws [64][17] (trtri scratch).
+__device__ void chain_run(const PobtafChain& P, double* S, double* Xs, double* Ws) {
+  const int tid = threadIdx.x;
+  double (*S2)[TT_LD] = reinterpret_cast<double (*)[TT_LD]>(S);
+  double (*X2)[TT_LD] = reinterpret_cast<double (*)[TT_LD]>(Xs);
+  double (*W2)[WS_LD] = reinterpret_cast<double (*)[WS_LD]>(Ws);
+  const int lane = tid & 63, wave = tid >> 6;
+  const int wm = (wave >> 1) * 32, wn = (wave & 1) * 32;
+  const int li = lane & 15, lk = lane >> 4;
+  for (int g = 0; g < P.NT; ++g) {
+    const CTile dt = tile_g(P, g, g);
+    f64x4 acc[2][2];
+    if (g > 0) chain_wait(P, P.hflags + t_pd(P, g), nullptr);
+    cacc_load(dt, acc);
+    if (g > 0) {
+      // acc -= L(g,g-1) L(g,g-1)^T, with L(g,g-1) in S as S[row][k]
+      if (wn <= wm) {
+#pragma unroll 4
+        for (int kk = 0; kk < CT; kk += 4) {
+          const double a0 = -S[(wm + li) * TT_LD + kk + lk];
+          const double a1 = -S[(wm + 16 + li) * TT_LD + kk + lk];
+          const double b0 = S[(wn + li) * TT_LD + kk + lk];
+          const double b1 = S[(wn + 16 + li) * TT_LD + kk + lk];
+          acc[0][0] = mfma_f64(a0, b0, acc[0][0]);
+          acc[0][1] = mfma_f64(a0, b1, acc[0][1]);
+          acc[1][0] = mfma_f64(a1, b0, acc[1][0]);
+          acc[1][1] = mfma_f64(a1, b1, acc[1][1]);
+        }
+      }
+      __syncthreads();
+    }
+    cacc_to_lds(acc, S, TT_LD);
+    __syncthreads();
+    const int m = dt.rows;
+    lds_potrf(S2, m, P.info, (int64_t)(g / P.nt) * P.b + (int64_t)(g % P.nt) * CT);
+    for (int idx = tid; idx < CT * CT; idx += CH_THREADS) {
+      const int rr = idx >> 6, cc = idx & 63;
+      if (rr < m && cc < m) dt.p[(int64_t)rr * dt.ld + cc] = (cc <= rr) ? S2[rr][cc] : 0.0;
+    }
+    lds_trtri(S2, X2, W2, m);
+    double* li_out = P.linv + (int64_t)g * CT * CT;
+    for (int idx = tid; idx < CT * CT; idx += CH_THREADS) li_out[idx] = X2[idx >> 6][idx & 63];
+    chain_publish(P.cflags + 2 * g);
+    if (g + 1 >= band_end(P, g)) continue;
+    // L(g+1,g) = PX(g) * Linv(g)^T
+    chain_wait(P, P.hflags + t_px(P, g), nullptr);
+    const CTile xt = tile_g(P, g + 1, g);
+    {
+      TileRegs16 rx;
+      tile_load16(xt, rx);
+      // S[m][k] = X(m, k) (pitch 65 rows are 8-byte aligned only: scalar stores)
+      const int rs = tid >> 5, c = (tid & 31) * 2;
+#pragma unroll
+      for (int q = 0; q < 8; ++q) {
+        S[(rs + 8 * q) * TT_LD + c] = rx.v[q].x;
+        S[(rs + 8 * q) * TT_LD + c + 1] = rx.v[q].y;
+      }
+    }
+    __syncthreads();
+    cacc_zero(acc);
+#pragma unroll 4
+    for (int kk = 0; kk < CT; kk += 4) {
+      const double a0 = S[(wm + li) * TT_LD + kk + lk];
+      const double a1 = S[(wm + 16 + li) * TT_LD + kk + lk];
+      const double b0 = Xs[(wn + li) * TT_LD + kk + lk];  // Linv(n, k)
+      const double b1 = Xs[(wn + 16 + li) * TT_LD + kk + lk];
+      acc[0][0] = mfma_f64(a0, b0, acc[0][0]);
+      acc[0][1] = mfma_f64(a0, b1, acc[0][1]);
+      acc[1][0] = mfma_f64(a1, b0, acc[1][0]);
+      acc[1][1] = mfma_f64(a1, b1, acc[1][1]);
+    }
+    cacc_store(xt, acc);
+    __syncthreads();  // S no longer read as X
+    cacc_to_lds(acc, S, TT_LD);  // keep L(g+1,g) as S[row][k]
+    chain_publish(P.cflags + 2 * g + 1);
+  }
+}
+
+// ---------------------------------------------------------------- kernel
+__global__ void __launch_bounds__(CH_THREADS) pobtaf_chain_kernel(PobtafChain P) {
+  __shared__ __attribute__((aligned(16))) double lds[2 * CT * KP + CT * WS_LD];
+  __shared__ int s_task;
+  const int tid = threadIdx.x;
+  if (tid == 0) s_task = atomicAdd(P.claim, 1);
+  __syncthreads();
+  const bool is_chain = (s_task == 0);
+  __syncthreads();
+  if (is_chain) {
+    chain_run(P, lds, lds + CT * TT_LD, lds + 2 * CT * TT_LD);
+    return;
+  }
+  double* XA = lds;
+  double* XB = lds + CT * KP;
+  for (;;) {
+    if (tid == 0) s_task = atomicAdd(P.counter, 1);
+    __syncthreads();
+    const int task = s_task;
+    __syncthreads();
+    if (task >= P.ntasks) return;
+    int r, j;
+    const int kind = chain_decode(P, task, r, j);
+    if (kind == HK_NOOP) continue;  // nobody waits on an empty slot
+
+    f64x4 acc[2][2];
+    if (kind == HK_TIP) {
+      cacc_load({P.T, P.a, P.a, P.a}, acc);
+      for (int i = 0; i < P.n; ++i) {
+        chain_wait(P, P.hflags + t_tippc(P, i), nullptr);
+        f64x4 pi[2][2];
+        cacc_load({P.tipp + (int64_t)i * P.a * P.a, P.a, P.a, P.a}, pi);
+#pragma unroll
+        for (int x = 0; x < 2; ++x)
+#pragma unroll
+          for (int y = 0; y < 2; ++y) acc[x][y] -= pi[x][y];
+      }
+      double (*S2)[TT_LD] = reinterpret_cast<double (*)[TT_LD]>(lds);
+      cacc_to_lds(acc, lds, TT_LD);
+      __syncthreads();
+      lds_potrf(S2, P.a, P.info, (int64_t)P.n * P.b);
+      for (int idx = tid; idx < P.a * P.a; idx += CH_THREADS) {
+        const int rr = idx / P.a, cc = idx % P.a;
+        P.T[idx] = (cc <= rr) ? S2[rr][cc] : 0.0;
+      }
+      chain_publish(P.hflags + task);
+      continue;
+    }
+    if (kind == HK_TIPP) {
+      const int i = j;
+      cacc_zero(acc);
+      for (int c = 0; c < P.nt; ++c) {
+        const int k = i * P.nt + c;
+        chain_wait(P, P.hflags + t_arr(P, k), nullptr);
+        TileRegs16 x;
+        tile_load16(tile_arrow(P, k), x);
+        __syncthreads();
+        tile_store16(x, XA);
+        __syncthreads();
+        mfma_abt(XA, XA, acc, 1.0, true);
+      }
+      cacc_store({P.tipp + (int64_t)i * P.a * P.a, P.a, P.a, P.a}, acc);
+      chain_publish(P.hflags + task);
+      continue;
+    }
+    CTile out;
+    if (kind == HK_TRSM)
+      out = tile_g(P, r, j);
+    else if (kind == HK_ARR)
+      out = tile_arrow(P, j);
+    else if (kind == HK_PX)
+      out = tile_g(P, j + 1, j);
+    else
+      out = tile_g(P, j, j);  // PD
+    cacc_load(out, acc);
+    const ContribRange cr = chain_contribs(P, kind, r, j);
+    helper_accumulate(P, cr, -1.0, kind == HK_PD, XA, XB, acc);
+    if (kind == HK_TRSM || kind == HK_ARR) helper_solve(P, j, XA, XB, acc);
+    cacc_store(out, acc);
+    // the strict-upper mirror of a lower D-block tile is zero in the factor
+    if (kind == HK_TRSM && r / P.nt == j / P.nt) tile_zero(tile_mirror(P, r, j));
+    if (kind == HK_PX && (j + 1) / P.nt == j / P.nt) tile_zero(tile_mirror(P, j + 1, j));
+    chain_publish(P.hflags + task);
+  }
+}
+
+}  // namespace rocbta

@@ -14,7 +14,7 @@
 
 #include "../../include/rocbta.h"
 #include "gemm_f64.hpp"
-#include "pobtaf_dag.hpp"
+#include "pobtaf_chain.hpp"
 #include "solve_f64.hpp"
 #include "tile_f64.hpp"
 
@@ -316,22 +316,24 @@ SiLayout si_layout(int64_t n, int64_t b, int64_t a) {
   l.w = take((size_t)nch * ntb * TT * TT);
   l.tinv = take((size_t)a * a);
   l.wt = take((size_t)nta * TT * TT);
-  l.part = take((size_t)ntb * ntb * 2 * TT * TT);  // split-K partial tiles (2 splits)
-  l.cnt = take((size_t)(ntb * ntb + 64) / 2 + 1);  // split-K tile counters (ints)
+  l.part = take((size_t)(ntb * ntb + ntb) * 2 * TT * TT);  // split-K partial tiles (2 splits)
+  l.cnt = take((size_t)(ntb * ntb + ntb + 64) / 2 + 1);  // split-K tile counters (ints)
   l.total = off;
   return l;
 }
 
 struct AfLayout {
-  size_t tinv, counter, flags, linv, tipp, total;
-  int ntasks, tpb;
+  size_t tinv, counter, claim, hflags, cflags, linv, tipp, total;
+  int nt, NT, slots, nstages, ntasks;
 };
 
 AfLayout af_layout(int64_t n, int64_t b, int64_t a) {
   AfLayout l{};
-  const int64_t nt = (b + DT - 1) / DT, ha = a > 0 ? 1 : 0;
-  l.tpb = (int)(nt * (2 * nt + ha) - (nt * (nt - 1)) / 2 + ha);
-  l.ntasks = (int)(n * l.tpb + ha);
+  l.nt = (int)((b + CT - 1) / CT);
+  l.NT = (int)(n * l.nt);
+  l.slots = 5 + 2 * l.nt;
+  l.nstages = l.NT + 2;
+  l.ntasks = l.nstages * l.slots;
   size_t off = 0;
   auto take = [&](size_t bytes) {
     size_t o = off;
@@ -340,8 +342,10 @@ AfLayout af_layout(int64_t n, int64_t b, int64_t a) {
   };
   l.tinv = take(TT * TT * sizeof(double));
   l.counter = take(sizeof(int));
-  l.flags = take((size_t)l.ntasks * sizeof(int));
-  l.linv = take((size_t)n * nt * DT * DT * sizeof(double));
+  l.claim = take(sizeof(int));
+  l.hflags = take((size_t)l.ntasks * sizeof(int));
+  l.cflags = take((size_t)2 * l.NT * sizeof(int));
+  l.linv = take((size_t)l.NT * CT * CT * sizeof(double));
   l.tipp = take((size_t)n * a * a * sizeof(double));
   l.total = off;
   return l;
@@ -361,16 +365,16 @@ int device_cus() {
 }
 
 // ROCBTA_POBTAF_ALGO=loop forces the launch-per-tile path (used by tests to
-// cover both paths); default: the persistent tile DAG whenever it applies.
-bool pobtaf_use_dag(int64_t a, int factorize_last_block) {
+// cover both paths); default: the persistent chain kernel whenever it applies.
+bool pobtaf_use_chain(int64_t a, int factorize_last_block) {
   const char* e = getenv("ROCBTA_POBTAF_ALGO");
   if (e && strcmp(e, "loop") == 0) return false;
-  return factorize_last_block && a <= DT;
+  return factorize_last_block && a <= CT;
 }
 
-int pobtaf_dag(int64_t n, int64_t b, int64_t a, double* D, double* L, double* R, double* T,
-               int* info, char* ws, const AfLayout& lay, hipStream_t s) {
-  PobtafDag P{};
+PobtafChain chain_params(int64_t n, int64_t b, int64_t a, double* D, double* L, double* R,
+                         double* T, int* info, char* ws, const AfLayout& lay) {
+  PobtafChain P{};
   P.D = D;
   P.L = L;
   P.R = R;
@@ -378,21 +382,33 @@ int pobtaf_dag(int64_t n, int64_t b, int64_t a, double* D, double* L, double* R,
   P.n = (int)n;
   P.b = (int)b;
   P.a = (int)a;
-  P.nt = (int)((b + DT - 1) / DT);
+  P.nt = lay.nt;
   P.ha = a > 0 ? 1 : 0;
-  P.tpb = lay.tpb;
+  P.NT = lay.NT;
+  P.slots = lay.slots;
+  P.nstages = lay.nstages;
   P.ntasks = lay.ntasks;
-  P.counter = (int*)(ws + lay.counter);
-  P.flags = (int*)(ws + lay.flags);
+  if (ws) {
+    P.counter = (int*)(ws + lay.counter);
+    P.claim = (int*)(ws + lay.claim);
+    P.hflags = (int*)(ws + lay.hflags);
+    P.cflags = (int*)(ws + lay.cflags);
+    P.linv = (double*)(ws + lay.linv);
+    P.tipp = (double*)(ws + lay.tipp);
+  }
   P.info = info;
-  P.linv = (double*)(ws + lay.linv);
-  P.tipp = (double*)(ws + lay.tipp);
   P.spin_limit = 1ll << 22;
+  return P;
+}
+
+int pobtaf_chain(int64_t n, int64_t b, int64_t a, double* D, double* L, double* R, double* T,
+                 int* info, char* ws, const AfLayout& lay, hipStream_t s) {
+  PobtafChain P = chain_params(n, b, a, D, L, R, T, info, ws, lay);
   const int64_t nflag = (int64_t)((lay.linv - lay.counter) / sizeof(int));
   fill_i32_kernel<<<(int)std::min<int64_t>(1024, (nflag + 255) / 256), 256, 0, s>>>(P.counter, nflag, 0);
   RB_CHECK(hipGetLastError());
-  const int grid = std::max(1, std::min(P.ntasks, 2 * device_cus()));
-  pobtaf_dag_kernel<<<grid, DAG_THREADS, 0, s>>>(P);
+  const int grid = std::max(2, std::min(P.ntasks + 1, 2 * device_cus()));
+  pobtaf_chain_kernel<<<grid, CH_THREADS, 0, s>>>(P);
   RB_CHECK(hipGetLastError());
   return ROCBTA_OK;
 }
@@ -436,8 +452,8 @@ int rocbta_dpobtaf(int64_t n, int64_t b, int64_t a, double* D, double* L, double
   if (!info) return ROCBTA_ERR_ARG;
   set_int_kernel<<<1, 64, 0, s>>>(info, 0);
   RB_CHECK(hipGetLastError());
-  if (pobtaf_use_dag(a, factorize_last_block))
-    return pobtaf_dag(n, b, a, D, L, R, T, info, (char*)workspace, lay, s);
+  if (pobtaf_use_chain(a, factorize_last_block))
+    return pobtaf_chain(n, b, a, D, L, R, T, info, (char*)workspace, lay, s);
   const int bi = (int)b, ai = (int)a;
   const int64_t bb = b * b, ab = a * b;
   for (int64_t i = 0; i + 1 < n; ++i) {
@@ -501,9 +517,12 @@ int rocbta_dpobtasi(int64_t n, int64_t b, int64_t a, double* D, double* L, doubl
   // product alone has too few tiles to give every CU two workgroups.
   const int ksplit = (ntb * ntb < 2 * device_cus() && b >= 256) ? 2 : 1;
   if (ksplit > 1) {
-    fill_i32_kernel<<<1, 256, 0, s>>>(cnt, ntb * ntb, 0);
+    fill_i32_kernel<<<1, 256, 0, s>>>(cnt, ntb * ntb + ntb, 0);
     RB_CHECK(hipGetLastError());
   }
+  // the arrow problem of a step has its own split-K bookkeeping
+  double* part_r = part + (int64_t)ntb * ntb * 2 * TT * TT;
+  int* cnt_r = cnt + ntb * ntb;
   const int bi = (int)b, ai = (int)a;
   const int64_t bb = b * b, ab = a * b;
   // Blocks whose factor is still stored: all of them, or all but the last
@@ -518,13 +537,16 @@ int rocbta_dpobtasi(int64_t n, int64_t b, int64_t a, double* D, double* L, doubl
   const int nchunks = (npre + SI_CHUNK - 1) / SI_CHUNK;
   SideStream* side = side_stream(nchunks + 1);
   if (!side) return ROCBTA_ERR_HIP;
+  // ROCBTA_SI_OVERLAP=0 runs the precompute on the caller's stream first.
+  const char* ov = getenv("ROCBTA_SI_OVERLAP");
+  const hipStream_t ps = (ov && strcmp(ov, "0") == 0) ? s : side->s;
   RB_CHECK(hipEventRecord(side->ev[0], s));
-  RB_CHECK(hipStreamWaitEvent(side->s, side->ev[0], 0));
+  RB_CHECK(hipStreamWaitEvent(ps, side->ev[0], 0));
   for (int k = 0; k < nchunks; ++k) {
     const int hi = npre - k * SI_CHUNK;             // exclusive
     const int lo = std::max(0, hi - SI_CHUNK);      // inclusive
-    const int cnt = hi - lo;
-    int rc = block_trtri(D + lo * bb, b, bb, Linv, b, bb, bi, cnt, W, side->s);
+    const int nblk = hi - lo;
+    int rc = block_trtri(D + lo * bb, b, bb, Linv, b, bb, bi, nblk, W, ps);
     if (rc) return rc;
     GemmList pre;
     const int ng = std::min<int>(hi, (int)n - 1) - lo;
@@ -532,15 +554,15 @@ int rocbta_dpobtasi(int64_t n, int64_t b, int64_t a, double* D, double* L, doubl
       push_term(pre.add(G + lo * bb, b, bi, bi, 0.0, TRI_NONE, ng, bb),
                 term(L + lo * bb, b, false, Linv, b, false, bi, 1.0, TRI_NONE, TRI_LOWER, bb, bb));
     if (a > 0)
-      push_term(pre.add(Gt + lo * ab, b, ai, bi, 0.0, TRI_NONE, cnt, ab),
+      push_term(pre.add(Gt + lo * ab, b, ai, bi, 0.0, TRI_NONE, nblk, ab),
                 term(R + lo * ab, b, false, Linv, b, false, bi, 1.0, TRI_NONE, TRI_LOWER, ab, bb));
     // S_i is symmetric: lower tiles computed, mirrored to the upper part
-    GemmProblem& ps = pre.add(D + lo * bb, b, bi, bi, 0.0, TRI_LOWER, cnt, bb);
-    ps.mirror = 1;
-    push_term(ps, term(Linv, b, true, Linv, b, false, bi, 1.0, TRI_UPPER, TRI_LOWER, bb, bb));
-    rc = run_gemms(pre, side->s);
+    GemmProblem& psym = pre.add(D + lo * bb, b, bi, bi, 0.0, TRI_LOWER, nblk, bb);
+    psym.mirror = 1;
+    push_term(psym, term(Linv, b, true, Linv, b, false, bi, 1.0, TRI_UPPER, TRI_LOWER, bb, bb));
+    rc = run_gemms(pre, ps);
     if (rc) return rc;
-    RB_CHECK(hipEventRecord(side->ev[k + 1], side->s));
+    RB_CHECK(hipEventRecord(side->ev[k + 1], ps));
   }
   auto chunk_of = [&](int64_t blk) { return (int)((npre - 1 - blk) / SI_CHUNK); };
   int waited = -1;
@@ -592,6 +614,9 @@ int rocbta_dpobtasi(int64_t n, int64_t b, int64_t a, double* D, double* L, doubl
     if (a > 0) {
       push_term(pl, term(R + (i + 1) * ab, b, true, Gt + i * ab, b, false, ai, -1.0));
       GemmProblem& pr = off.add(R + i * ab, b, ai, bi, 0.0);
+      pr.ksplit = ksplit;
+      pr.partial = part_r;
+      pr.counters = cnt_r;
       push_term(pr, term(R + (i + 1) * ab, b, false, G + i * bb, b, false, bi, -1.0));
       push_term(pr, term(T, a, false, Gt + i * ab, b, false, ai, -1.0));
     }
@@ -743,64 +768,104 @@ int rocbta_dpobtas(int64_t n, int64_t b, int64_t a, const double* D, const doubl
 }
 
 int64_t rocbta_dag_selfcheck(int64_t n, int64_t b, int64_t a) {
-  if (bad_shape(n, b, a) || a > DT) return -1;
+  if (bad_shape(n, b, a) || a > CT) return -1;
   const AfLayout lay = af_layout(n, b, a);
-  PobtafDag P{};
   double* const base = reinterpret_cast<double*>(uintptr_t(1) << 40);
-  P.D = base;
-  P.L = base + ((int64_t)1 << 36);
-  P.R = base + ((int64_t)2 << 36);
-  P.T = base + ((int64_t)3 << 36);
-  P.n = (int)n;
-  P.b = (int)b;
-  P.a = (int)a;
-  P.nt = (int)((b + DT - 1) / DT);
-  P.ha = a > 0 ? 1 : 0;
-  P.tpb = lay.tpb;
-  P.ntasks = lay.ntasks;
+  int* const fbase = reinterpret_cast<int*>(uintptr_t(1) << 44);
+  PobtafChain P = chain_params(n, b, a, base, base + ((int64_t)1 << 36), base + ((int64_t)2 << 36),
+                               base + ((int64_t)3 << 36), nullptr, nullptr, lay);
+  P.hflags = fbase;
+  P.cflags = fbase + ((int64_t)1 << 32);
   int64_t bad = 0;
-  std::vector<char> seen((size_t)P.ntasks, 0);
-  auto in_range = [&](const DagTile& t, const double* arr, int64_t count) {
+  auto in_range = [&](const CTile& t, const double* arr, int64_t count) {
     const int64_t off = t.p - arr;
-    if (t.rows <= 0 || t.cols <= 0 || t.rows > DT || t.cols > DT) return false;
+    if (t.rows <= 0 || t.cols <= 0 || t.rows > CT || t.cols > CT) return false;
     return off >= 0 && off + (int64_t)(t.rows - 1) * t.ld + (t.cols - 1) < count;
   };
-  auto tile_ok = [&](const DagTile& t) {
+  auto tile_ok = [&](const CTile& t) {
     return in_range(t, P.D, n * b * b) || in_range(t, P.L, (n - 1) * b * b) ||
            in_range(t, P.R, n * a * b);
   };
+  // a helper may wait on a helper task strictly earlier in the queue, or on a
+  // chain output of a step before its stage
+  auto wait_ok = [&](const int* f, int task) {
+    const int stage = task / P.slots;
+    if (f >= P.cflags) {
+      const int64_t idx = f - P.cflags;
+      return idx >= 0 && idx < 2 * (int64_t)P.NT && idx / 2 <= stage - 1;
+    }
+    const int64_t idx = f - P.hflags;
+    return idx >= 0 && idx < task;
+  };
+  std::vector<int> seen_trsm, seen_other;
+  int64_t nwork = 0;
   for (int t = 0; t < P.ntasks; ++t) {
-    int kind, i, r, c;
-    dag_decode(P, t, kind, i, r, c);
-    int back = -1;
-    if (kind == K_DTILE) back = (i < P.n && c <= r && r < P.nt) ? t_offd(P, i, r, c) : -1;
-    if (kind == K_LOW) back = (i < P.n && r < P.nt && c < P.nt) ? t_low(P, i, r, c) : -1;
-    if (kind == K_ARR) back = (P.ha && i < P.n && c < P.nt) ? t_arr(P, i, c) : -1;
-    if (kind == K_TIPP) back = (P.ha && i < P.n) ? t_tipp(P, i) : -1;
-    if (kind == K_TIP) back = P.n * P.tpb;
-    if (back != t || seen[t]) ++bad;
-    seen[t] = 1;
-    if (kind == K_TIP) {
-      for (int ii = 0; ii < P.n; ++ii)
-        if (t_tipp(P, ii) >= t) ++bad;
+    int r, j;
+    const int kind = chain_decode(P, t, r, j);
+    if (kind == HK_NOOP) continue;
+    ++nwork;
+    if (kind == HK_TIP) {
+      if (t != t_tipc(P)) ++bad;
+      for (int i = 0; i < P.n; ++i)
+        if (!wait_ok(P.hflags + t_tippc(P, i), t)) ++bad;
       continue;
     }
-    if (kind == K_LOW && i == P.n - 1) continue;
-    Contrib cb;
-    for (int j = 0; dag_contrib(P, kind, i, r, c, j, cb); ++j) {
-      if (cb.fa < 0 || cb.fa >= t || cb.fb >= t) ++bad;
-      if (!tile_ok(cb.A) || !tile_ok(cb.B)) ++bad;
-      if (j > 2 * P.nt) {
-        ++bad;
-        break;
-      }
+    if (kind == HK_TIPP) {
+      if (t != t_tippc(P, j)) ++bad;
+      for (int c = 0; c < P.nt; ++c)
+        if (!wait_ok(P.hflags + t_arr(P, j * P.nt + c), t)) ++bad;
+      continue;
     }
-    if ((kind == K_DTILE && r > c) || kind == K_LOW || kind == K_ARR)
-      if (t_diag(P, i, c) >= t) ++bad;
-    if (kind == K_DTILE && !tile_ok(tile_D(P, i, r, c))) ++bad;
-    if (kind == K_LOW && !tile_ok(tile_L(P, i, r, c))) ++bad;
-    if (kind == K_ARR && !tile_ok(tile_R(P, i, c))) ++bad;
+    int back = -1;
+    if (kind == HK_TRSM) back = t_trsm(P, r, j);
+    if (kind == HK_ARR) back = t_arr(P, j);
+    if (kind == HK_PX) back = t_px(P, j);
+    if (kind == HK_PD) back = t_pd(P, j);
+    if (back != t) ++bad;
+    if (kind == HK_TRSM && !(r >= j + 2 && r < band_end(P, j))) ++bad;
+    const ContribRange cr = chain_contribs(P, kind, r, j);
+    for (int k = cr.k0; k < cr.k1; ++k) {
+      const CTile ta = cr.ra < 0 ? tile_arrow(P, k) : tile_g(P, cr.ra, k);
+      if (!tile_ok(ta) || !tile_ok(tile_g(P, cr.jb, k))) ++bad;
+      if (!wait_ok(factor_flag(P, cr.ra, k), t) || !wait_ok(factor_flag(P, cr.jb, k), t)) ++bad;
+      if (cr.ra >= 0 && !(cr.ra > k)) ++bad;
+      if (!(cr.jb > k)) ++bad;
+    }
+    if ((kind == HK_TRSM || kind == HK_ARR) && !wait_ok(P.cflags + 2 * j, t)) ++bad;
+    CTile out = kind == HK_TRSM ? tile_g(P, r, j)
+                : kind == HK_ARR ? tile_arrow(P, j)
+                : kind == HK_PX  ? tile_g(P, j + 1, j)
+                                 : tile_g(P, j, j);
+    if (!tile_ok(out)) ++bad;
+    if (kind == HK_TRSM && r / P.nt == j / P.nt && !tile_ok(tile_mirror(P, r, j))) ++bad;
   }
+  // chain: step g waits PD(g) (g >= 1) and PX(g) when a sub-diagonal exists;
+  // each must be a decoded task of the right kind, and it may only depend on
+  // chain steps < g, which wait_ok enforces through its stage.
+  for (int g = 0; g < P.NT; ++g) {
+    int r, j;
+    if (g > 0) {
+      const int tpd = t_pd(P, g);
+      if (chain_decode(P, tpd, r, j) != HK_PD || j != g || tpd / P.slots > g) ++bad;
+    }
+    if (g + 1 < band_end(P, g)) {
+      const int tpx = t_px(P, g);
+      if (chain_decode(P, tpx, r, j) != HK_PX || j != g || tpx / P.slots > g) ++bad;
+      if (!tile_ok(tile_g(P, g + 1, g))) ++bad;
+    }
+    if (!tile_ok(tile_g(P, g, g))) ++bad;
+  }
+  // every lower band tile is produced exactly once (chain or helper)
+  int64_t expect = 0;
+  for (int j = 0; j < P.NT; ++j) expect += std::max(0, band_end(P, j) - (j + 2)) + (P.ha ? 1 : 0);
+  int64_t got = 0;
+  for (int t = 0; t < P.ntasks; ++t) {
+    int r, j;
+    const int kind = chain_decode(P, t, r, j);
+    if (kind == HK_TRSM || kind == HK_ARR) ++got;
+  }
+  if (got != expect) ++bad;
+  (void)nwork;
   return bad;
 }
 

@@ -45,6 +45,26 @@ __device__ __forceinline__ double readlane_f64(double v, int lane) {
   return __builtin_bit_cast(double, ((uint64_t)hi << 32) | lo);
 }
 
+// d = sqrt(x), inv = 1/sqrt(x) for x > 0 (NaN propagates), from the hardware
+// reciprocal square root refined by Newton's iteration for 1/sqrt,
+//   y <- y + y (1 - x y^2) / 2,
+// twice (quadratic convergence from the ~2^-26 seed), then one Newton step
+// on d = x y for the square root itself, d <- d + y (x - d^2) / 2.  The
+// dependent chain is about a third of sqrt() followed by a division.
+__device__ __forceinline__ void pivot_sqrt(double x, double& d, double& inv) {
+  double y = __builtin_amdgcn_rsq(x);
+  const double h = 0.5 * x;
+  double e = __builtin_fma(-h * y, y, 0.5);
+  y = __builtin_fma(y, e, y);
+  e = __builtin_fma(-h * y, y, 0.5);
+  y = __builtin_fma(y, e, y);
+  double s = x * y;
+  const double rr = __builtin_fma(-s, s, x);
+  s = __builtin_fma(0.5 * y, rr, s);
+  d = s;
+  inv = y;
+}
+
 // Unblocked Cholesky of the panel S[p:m, p:p+w] (w <= 16, m - p <= 64) by
 // the calling wave; lane l owns row p + l.  Lower entries are written back.
 __device__ __forceinline__ void wave_panel_potrf(double (*S)[TT_LD], int p, int m, int w,
@@ -63,8 +83,8 @@ __device__ __forceinline__ void wave_panel_potrf(double (*S)[TT_LD], int p, int 
         if (lane == 0) record_info(info, (int)(row0 + p + j + 1));
         piv = __builtin_nan("");
       }
-      const double d = sqrt(piv);
-      const double inv = 1.0 / d;
+      double d, inv;
+      pivot_sqrt(piv, d, inv);
       r[j] = (lane == j) ? d : (lane > j ? r[j] * inv : r[j]);
 #pragma unroll
       for (int k = j + 1; k < PW; ++k) {
