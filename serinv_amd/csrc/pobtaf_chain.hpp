@@ -52,10 +52,19 @@ struct PobtafChain {
   int* hflags;   // helper task flags
   int* cflags;   // chain flags: [2g] Linv(g) ready, [2g+1] L(g+1,g) ready
   int* info;
-  double* linv;  // NT * 64 * 64
-  double* tipp;  // n * a * a
+  int* chain_cu;  // 1 + hardware CU key of the chain workgroup (0 until known)
+  double* linv;   // NT * 64 * 64
+  double* tipp;   // n * a * a
   long long spin_limit;
 };
+
+// Hardware key of the CU running the calling wave: XCC id and the
+// SE/SH/CU fields of HW_ID.
+__device__ __forceinline__ int hw_cu_key() {
+  const unsigned hw = __builtin_amdgcn_s_getreg((31 << 11) | (0 << 6) | 4);     // HW_ID
+  const unsigned xcc = __builtin_amdgcn_s_getreg((3 << 11) | (0 << 6) | 20);    // XCC_ID
+  return (int)(((xcc & 15u) << 8) | ((hw >> 8) & 0xffu));
+}
 
 struct CTile {
   double* p;
@@ -482,13 +491,20 @@ __global__ void __launch_bounds__(CH_THREADS) pobtaf_chain_kernel(PobtafChain P)
   const bool is_chain = (s_task == 0);
   __syncthreads();
   if (is_chain) {
+    // the chain runs latency-bound code: keep helpers off its CU
+    if (tid == 0)
+      __hip_atomic_store(P.chain_cu, 1 + hw_cu_key(), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     chain_run(P, lds, lds + CT * TT_LD, lds + 2 * CT * TT_LD);
     return;
   }
   double* XA = lds;
   double* XB = lds + CT * KP;
+  const int my_cu = 1 + hw_cu_key();
   for (;;) {
-    if (tid == 0) s_task = atomicAdd(P.counter, 1);
+    if (tid == 0) {
+      const int ccu = __hip_atomic_load(P.chain_cu, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      s_task = (ccu == my_cu) ? P.ntasks : atomicAdd(P.counter, 1);
+    }
     __syncthreads();
     const int task = s_task;
     __syncthreads();

@@ -323,7 +323,7 @@ SiLayout si_layout(int64_t n, int64_t b, int64_t a) {
 }
 
 struct AfLayout {
-  size_t tinv, counter, claim, hflags, cflags, linv, tipp, total;
+  size_t tinv, counter, claim, chain_cu, hflags, cflags, linv, tipp, total;
   int nt, NT, slots, nstages, ntasks;
 };
 
@@ -343,6 +343,7 @@ AfLayout af_layout(int64_t n, int64_t b, int64_t a) {
   l.tinv = take(TT * TT * sizeof(double));
   l.counter = take(sizeof(int));
   l.claim = take(sizeof(int));
+  l.chain_cu = take(sizeof(int));
   l.hflags = take((size_t)l.ntasks * sizeof(int));
   l.cflags = take((size_t)2 * l.NT * sizeof(int));
   l.linv = take((size_t)l.NT * CT * CT * sizeof(double));
@@ -391,6 +392,7 @@ PobtafChain chain_params(int64_t n, int64_t b, int64_t a, double* D, double* L, 
   if (ws) {
     P.counter = (int*)(ws + lay.counter);
     P.claim = (int*)(ws + lay.claim);
+    P.chain_cu = (int*)(ws + lay.chain_cu);
     P.hflags = (int*)(ws + lay.hflags);
     P.cflags = (int*)(ws + lay.cflags);
     P.linv = (double*)(ws + lay.linv);

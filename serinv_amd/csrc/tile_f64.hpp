@@ -67,8 +67,10 @@ __device__ __forceinline__ void pivot_sqrt(double x, double& d, double& inv) {
 
 // Unblocked Cholesky of the panel S[p:m, p:p+w] (w <= 16, m - p <= 64) by
 // the calling wave; lane l owns row p + l.  Lower entries are written back.
-__device__ __forceinline__ void wave_panel_potrf(double (*S)[TT_LD], int p, int m, int w,
+template <int WFIX>
+__device__ __forceinline__ void wave_panel_potrf(double (*S)[TT_LD], int p, int m, int w_rt,
                                                  int* info, int64_t row0) {
+  const int w = WFIX ? WFIX : w_rt;
   const int lane = threadIdx.x & 63;
   const int i = p + lane;
   const bool act = i < m;
@@ -102,7 +104,10 @@ __device__ __forceinline__ void wave_panel_potrf(double (*S)[TT_LD], int p, int 
 
 // S[q:m, q:m] (lower part) -= P P^T with P = S[q:m, p:p+w], on MFMA, all
 // waves.  Output 16x16 blocks are dealt round-robin to the waves.
-__device__ __forceinline__ void lds_syrk_update(double (*S)[TT_LD], int p, int w, int q, int m) {
+template <int WFIX>
+__device__ __forceinline__ void lds_syrk_update(double (*S)[TT_LD], int p, int w_rt, int q,
+                                                int m) {
+  const int w = WFIX ? WFIX : w_rt;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int nb = (m - q + 15) / 16;
   const int li = lane & 15, lk = lane >> 4;
@@ -128,18 +133,38 @@ __device__ __forceinline__ void lds_syrk_update(double (*S)[TT_LD], int p, int w
     }
 }
 
-// Blocked Cholesky of S (m x m, lower part valid) in place.
-__device__ void lds_potrf(double (*S)[TT_LD], int m, int* info, int64_t row0) {
+// Blocked Cholesky of S (m x m, lower part valid) in place.  MFIX = 64
+// compiles the full-tile case with all sizes constant; MFIX = 0 handles any
+// m <= 64.
+template <int MFIX>
+__device__ __forceinline__ void lds_potrf_t(double (*S)[TT_LD], int m_rt, int* info,
+                                            int64_t row0) {
+  const int m = MFIX ? MFIX : m_rt;
   const int wave = threadIdx.x >> 6;
+#pragma unroll
   for (int p = 0; p < m; p += PW) {
     const int w = min(PW, m - p);
-    if (wave == 0) wave_panel_potrf(S, p, m, w, info, row0);
+    if (MFIX) {
+      if (wave == 0) wave_panel_potrf<PW>(S, p, m, PW, info, row0);
+    } else {
+      if (wave == 0) wave_panel_potrf<0>(S, p, m, w, info, row0);
+    }
     __syncthreads();
     if (p + w < m) {
-      lds_syrk_update(S, p, w, p + w, m);
+      if (MFIX)
+        lds_syrk_update<PW>(S, p, PW, p + w, m);
+      else
+        lds_syrk_update<0>(S, p, w, p + w, m);
       __syncthreads();
     }
   }
+}
+
+__device__ __forceinline__ void lds_potrf(double (*S)[TT_LD], int m, int* info, int64_t row0) {
+  if (m == TT)
+    lds_potrf_t<TT>(S, TT, info, row0);
+  else
+    lds_potrf_t<0>(S, m, info, row0);
 }
 
 // X = L^{-1} (m x m lower, in S) into Xs (lower; strict upper zero).
@@ -150,8 +175,10 @@ __device__ void lds_potrf(double (*S)[TT_LD], int m, int* info, int64_t row0) {
 // Ws: 64 x 17 scratch; at distance d the block (I, I-d) of W sits in rows
 // 16I .. 16I+15.
 constexpr int WS_LD = 17;
-__device__ void lds_trtri(const double (*S)[TT_LD], double (*Xs)[TT_LD], double (*Ws)[WS_LD],
-                          int m) {
+template <int MFIX>
+__device__ __forceinline__ void lds_trtri_t(const double (*S)[TT_LD], double (*Xs)[TT_LD],
+                                            double (*Ws)[WS_LD], int m_rt) {
+  const int m = MFIX ? MFIX : m_rt;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int tid = threadIdx.x;
   for (int idx = tid; idx < TT * TT; idx += TT_THREADS) Xs[idx / TT][idx % TT] = 0.0;
@@ -221,6 +248,14 @@ __device__ void lds_trtri(const double (*S)[TT_LD], double (*Xs)[TT_LD], double 
     }
     __syncthreads();
   }
+}
+
+__device__ __forceinline__ void lds_trtri(const double (*S)[TT_LD], double (*Xs)[TT_LD],
+                                          double (*Ws)[WS_LD], int m) {
+  if (m == TT)
+    lds_trtri_t<TT>(S, Xs, Ws, TT);
+  else
+    lds_trtri_t<0>(S, Xs, Ws, m);
 }
 
 // mode 0: potrf (+ inverse into X if X != nullptr); mode 1: trtri only.
