@@ -66,7 +66,10 @@ int rocbta_dpobtasi(int64_t n, int64_t b, int64_t a, double* D, double* L, doubl
  * trans 'N': B <- L^{-1} B ; 'T' or 'C': B <- L^{-T} B.
  * Replaces serinv.algs.pobtas / _pobtas (src/serinv/algs/pobtas.py:10-62,
  * 65-161) and, with a == 0, serinv.algs.pobts / _pobts (pobts.py:10-118).
- * partial = 1 skips the last-block and tip solves (pobtas.py:99,129). */
+ * partial = 1 skips the last-block and tip solves (pobtas.py:99,129).
+ * With partial = 0 the solve runs as one persistent kernel per call; the
+ * first int32 of the workspace then holds 0, or -1 if that kernel hit its
+ * internal scheduling time-out (the contents of B are then undefined). */
 int rocbta_dpobtas(int64_t n, int64_t b, int64_t a, const double* D, const double* L,
                    const double* R, const double* T, double* B, int64_t r, int64_t ldb,
                    char trans, int partial, void* workspace, size_t workspace_bytes,

@@ -16,6 +16,7 @@
 #include "gemm_f64.hpp"
 #include "pobtaf_chain.hpp"
 #include "solve_f64.hpp"
+#include "solve_sweep.hpp"
 #include "tile_f64.hpp"
 
 #include <cstdlib>
@@ -415,6 +416,139 @@ int pobtaf_chain(int64_t n, int64_t b, int64_t a, double* D, double* L, double* 
   return ROCBTA_OK;
 }
 
+// ------------------------------------------------ persistent solve sweep
+struct SwLayout {
+  size_t status, claim, counter, tinv, ybuf, part, apart, total;
+  int nt, ta, NT, NG, rc, nchunks, ntasks;
+};
+
+int sweep_rc(int64_t r) { return r <= 1 ? 1 : (r <= 4 ? 4 : 8); }
+
+SwLayout sw_layout(int64_t n, int64_t b, int64_t a, int64_t r) {
+  SwLayout l{};
+  l.nt = cdiv(b, SW_T);
+  l.ta = cdiv(a, SW_T);
+  l.NT = (int)(n * l.nt);
+  l.NG = l.NT + l.ta;
+  l.rc = sweep_rc(r);
+  l.nchunks = std::max(1, cdiv(r, l.rc));
+  l.ntasks = 2 * l.NT * l.nchunks;
+  size_t off = 0;
+  auto take = [&](size_t bytes) {
+    size_t o = off;
+    off = align256(off + bytes);
+    return o;
+  };
+  // status first: the caller may read it at workspace offset 0
+  l.status = take(sizeof(int));
+  l.claim = take(sizeof(int));
+  l.counter = take(sizeof(int));
+  l.tinv = take((size_t)l.NG * SW_T * SW_T * sizeof(double));
+  // published blocks (pre-filled with the empty marker), contiguous
+  l.ybuf = take((size_t)l.nchunks * l.NG * SW_T * l.rc * sizeof(double));
+  l.part = take((size_t)l.nchunks * l.NG * SW_T * l.rc * sizeof(double));
+  l.apart = take((size_t)l.nchunks * n * l.ta * SW_T * l.rc * sizeof(double));
+  l.total = off;
+  return l;
+}
+
+__global__ void fill_u64_kernel(unsigned long long* p, int64_t count, unsigned long long v) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (; i < count; i += stride) p[i] = v;
+}
+
+// ROCBTA_POBTAS_ALGO=loop forces the launch-per-tile solve (tests cover both).
+bool pobtas_use_sweep(int partial) {
+  const char* e = getenv("ROCBTA_POBTAS_ALGO");
+  if (e && strcmp(e, "loop") == 0) return false;
+  return !partial;
+}
+
+// Inverses of the diagonal tiles of an m_total x m_total lower-triangular
+// matrix stored with row stride ld, for `batch` matrices (stride sA), into
+// consecutive 64 x 64 tiles starting at X (per matrix: ceil(m_total/64) tiles).
+int diag_tile_inverses(const double* A, int64_t ld, int64_t sA, int m_total, int batch, double* X,
+                       int64_t sX, hipStream_t s) {
+  const int nfull = m_total / SW_T, rem = m_total % SW_T;
+  double* Am = const_cast<double*>(A);
+  if (nfull > 0) {
+    int rc = tile_factor(1, Am, ld, sA, (int64_t)SW_T * (ld + 1), X, SW_T, sX, SW_T * SW_T, SW_T,
+                         batch, nfull, nullptr, 0, 0, 0, 0, s);
+    if (rc) return rc;
+  }
+  if (rem > 0) {
+    int rc = tile_factor(1, Am + (int64_t)nfull * SW_T * (ld + 1), ld, sA, 0,
+                         X + (int64_t)nfull * SW_T * SW_T, SW_T, sX, 0, rem, batch, 1, nullptr,
+                         0, 0, 0, 0, s);
+    if (rc) return rc;
+  }
+  return ROCBTA_OK;
+}
+
+template <int RC>
+int launch_sweep(const SweepP& P, int grid, hipStream_t s) {
+  pobtas_sweep_kernel<RC><<<grid, SW_THREADS, 0, s>>>(P);
+  RB_CHECK(hipGetLastError());
+  return ROCBTA_OK;
+}
+
+int pobtas_sweep(int64_t n, int64_t b, int64_t a, const double* D, const double* L,
+                 const double* R, const double* T, double* B, int64_t r, int64_t ldb, int trans,
+                 char* ws, hipStream_t s) {
+  const SwLayout lay = sw_layout(n, b, a, r);
+  SweepP P{};
+  P.D = D;
+  P.L = L;
+  P.R = R;
+  P.T = T;
+  P.B = B;
+  P.ldb = ldb;
+  P.n = (int)n;
+  P.b = (int)b;
+  P.a = (int)a;
+  P.nt = lay.nt;
+  P.ta = lay.ta;
+  P.NT = lay.NT;
+  P.NG = lay.NG;
+  P.r = (int)r;
+  P.rc = lay.rc;
+  P.nchunks = lay.nchunks;
+  P.trans = trans;
+  P.tinv = (const double*)(ws + lay.tinv);
+  P.ybuf = (double*)(ws + lay.ybuf);
+  P.part = (double*)(ws + lay.part);
+  P.apart = (double*)(ws + lay.apart);
+  P.claim = (int*)(ws + lay.claim);
+  P.counter = (int*)(ws + lay.counter);
+  P.status = (int*)(ws + lay.status);
+  P.ntasks = lay.ntasks;
+  P.spin_limit = 1ll << 22;
+  double* tinv = (double*)(ws + lay.tinv);
+  int rc = diag_tile_inverses(D, b, b * b, (int)b, (int)n, tinv, (int64_t)lay.nt * SW_T * SW_T, s);
+  if (rc) return rc;
+  if (a > 0) {
+    rc = diag_tile_inverses(T, a, 0, (int)a, 1, tinv + (int64_t)lay.NT * SW_T * SW_T, 0, s);
+    if (rc) return rc;
+  }
+  const int64_t nflag = (int64_t)((lay.tinv - lay.status) / sizeof(int));
+  fill_i32_kernel<<<(int)std::min<int64_t>(1024, (nflag + 255) / 256), 256, 0, s>>>(P.status, nflag, 0);
+  RB_CHECK(hipGetLastError());
+  const int64_t nempty = (int64_t)((lay.total - lay.ybuf) / sizeof(double));
+  fill_u64_kernel<<<(int)std::min<int64_t>(4096, (nempty + 255) / 256), 256, 0, s>>>(
+      (unsigned long long*)(ws + lay.ybuf), nempty, SW_EMPTY);
+  RB_CHECK(hipGetLastError());
+  const int grid = std::max(P.nchunks + 1, std::min(P.nchunks + P.ntasks, 2 * device_cus()));
+  switch (lay.rc) {
+    case 1:
+      return launch_sweep<1>(P, grid, s);
+    case 4:
+      return launch_sweep<4>(P, grid, s);
+    default:
+      return launch_sweep<8>(P, grid, s);
+  }
+}
+
 bool bad_shape(int64_t n, int64_t b, int64_t a) {
   return n < 1 || b < 1 || a < 0 || b > (1 << 20) || a > (1 << 20) || n > (1 << 24);
 }
@@ -435,7 +569,7 @@ size_t rocbta_workspace_bytes(int op, int64_t n, int64_t b, int64_t a, int64_t r
     case ROCBTA_OP_POBTASI:
       return si_layout(n, b, a).total;
     case ROCBTA_OP_POBTAS:
-      return 256;
+      return sw_layout(n, b, a, std::max<int64_t>(r, 1)).total;
     default:
       return 0;
   }
@@ -713,14 +847,20 @@ int rocbta_dpobtas(int64_t n, int64_t b, int64_t a, const double* D, const doubl
                    const double* R, const double* T, double* B, int64_t r, int64_t ldb,
                    char trans, int partial, void* workspace, size_t workspace_bytes,
                    void* stream) {
-  (void)workspace;
-  (void)workspace_bytes;
   if (bad_shape(n, b, a) || !D || !B || (n > 1 && !L) || (a > 0 && (!R || !T)) || r < 0 ||
       ldb < r)
     return ROCBTA_ERR_ARG;
   if (trans != 'N' && trans != 'T' && trans != 'C') return ROCBTA_ERR_ARG;
   if (r == 0) return ROCBTA_OK;
   hipStream_t s = (hipStream_t)stream;
+  if (pobtas_use_sweep(partial)) {
+    if (!workspace || workspace_bytes < sw_layout(n, b, a, r).total) return ROCBTA_ERR_WORKSPACE;
+    return pobtas_sweep(n, b, a, D, L, R, T, B, r, ldb, trans == 'N' ? 0 : 1, (char*)workspace, s);
+  }
+  if (workspace && workspace_bytes >= sizeof(int)) {
+    set_int_kernel<<<1, 64, 0, s>>>((int*)workspace, 0);  // status word: no time-out
+    RB_CHECK(hipGetLastError());
+  }
   const int bi = (int)b, ai = (int)a, ri = (int)r;
   const int64_t bb = b * b, ab = a * b;
   double* Bt = B + n * b * ldb;  // tip rows

@@ -28,12 +28,14 @@ def algs(gpu_device):
 
 @pytest.fixture(params=["dag", "loop"])
 def pobtaf_algo(request, monkeypatch):
-    """Run the test with the persistent tile-DAG pobtaf and with the
-    launch-per-tile pobtaf (ROCBTA_POBTAF_ALGO=loop)."""
-    if request.param == "loop":
-        monkeypatch.setenv("ROCBTA_POBTAF_ALGO", "loop")
-    else:
-        monkeypatch.delenv("ROCBTA_POBTAF_ALGO", raising=False)
+    """Run the test with the persistent kernels (chain pobtaf, sweep pobtas)
+    and with the launch-per-tile forms (ROCBTA_POBTAF_ALGO=loop,
+    ROCBTA_POBTAS_ALGO=loop)."""
+    for var in ("ROCBTA_POBTAF_ALGO", "ROCBTA_POBTAS_ALGO"):
+        if request.param == "loop":
+            monkeypatch.setenv(var, "loop")
+        else:
+            monkeypatch.delenv(var, raising=False)
     return request.param
 
 
