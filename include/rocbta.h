@@ -81,6 +81,26 @@ int rocbta_dpobtas(int64_t n, int64_t b, int64_t a, const double* D, const doubl
 int rocbta_dlogdet(int64_t n, int64_t b, int64_t a, const double* D, const double* T,
                    double* out, void* stream);
 
+/* Gradient of f = gl * log det A + gq * sum_k y_k^T A^{-1} y_k with respect
+ * to the stored blocks of A (no counterpart in the reference; the backward of
+ * serinv_amd.autograd).  XD, XL, XR, XT: the selected inverse written by
+ * rocbta_dpobtasi; x = A^{-1} y, (n*b + a, r) with row stride ldx; gl, gq:
+ * DEVICE pointers to the two upstream scalars (NULL = 0).  Writes
+ *   GD, GT: w * (gl X - gq x x^T) with w = 2 below the diagonal, 1 on it,
+ *           0 above (only the lower triangle of D and T is read by pobtaf);
+ *   GL, GR: 2 * (gl X - gq x x^T) on the sub-diagonal and arrow blocks;
+ *   gy (optional, NULL to skip; row stride ldx): 2 gq x.
+ * G* may alias X* (in place); gy must not alias x. */
+int rocbta_dgrad(int64_t n, int64_t b, int64_t a, const double* XD, const double* XL,
+                 const double* XR, const double* XT, const double* x, int64_t r, int64_t ldx,
+                 const double* gl, const double* gq, double* GD, double* GL, double* GR,
+                 double* GT, double* gy, void* stream);
+
+/* *out = sum_{i < rows, k < r} y[i*ld + k] * x[i*ld + k] (device scalar;
+ * deterministic).  No counterpart in the reference. */
+int rocbta_dquad(int64_t rows, int64_t r, const double* y, const double* x, int64_t ld,
+                 double* out, void* stream);
+
 /* Diagnostic (no counterpart in the reference): runs the task decoding and
  * dependency enumeration of the persistent pobtaf kernel on the HOST for the
  * given shape and returns the number of violations (a task waiting on a task

@@ -40,6 +40,9 @@ SIGNATURES = {
                                 _c_i64, ctypes.c_char, _c_int, _c_p, _c_size, _c_p]),
     "rocbta_dlogdet": (_c_int, [_c_i64, _c_i64, _c_i64, _c_p, _c_p, _c_p, _c_p]),
     "rocbta_dag_selfcheck": (_c_i64, [_c_i64, _c_i64, _c_i64]),
+    "rocbta_dgrad": (_c_int, [_c_i64, _c_i64, _c_i64, _c_p, _c_p, _c_p, _c_p, _c_p, _c_i64, _c_i64,
+                              _c_p, _c_p, _c_p, _c_p, _c_p, _c_p, _c_p, _c_p]),
+    "rocbta_dquad": (_c_int, [_c_i64, _c_i64, _c_p, _c_p, _c_i64, _c_p, _c_p]),
 }
 
 _lib = None

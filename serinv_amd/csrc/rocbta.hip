@@ -14,6 +14,7 @@
 
 #include "../../include/rocbta.h"
 #include "gemm_f64.hpp"
+#include "grad_f64.hpp"
 #include "pobtaf_chain.hpp"
 #include "solve_f64.hpp"
 #include "solve_sweep.hpp"
@@ -1009,6 +1010,53 @@ int64_t rocbta_dag_selfcheck(int64_t n, int64_t b, int64_t a) {
   if (got != expect) ++bad;
   (void)nwork;
   return bad;
+}
+
+int rocbta_dgrad(int64_t n, int64_t b, int64_t a, const double* XD, const double* XL,
+                 const double* XR, const double* XT, const double* x, int64_t r, int64_t ldx,
+                 const double* gl, const double* gq, double* GD, double* GL, double* GR,
+                 double* GT, double* gy, void* stream) {
+  if (bad_shape(n, b, a) || !XD || !GD || (n > 1 && (!XL || !GL)) ||
+      (a > 0 && (!XR || !XT || !GR || !GT)) || r < 0 || (r > 0 && (!x || ldx < r)))
+    return ROCBTA_ERR_ARG;
+  if (gq && r == 0) return ROCBTA_ERR_ARG;
+  GradP P{};
+  P.XD = XD;
+  P.XL = XL;
+  P.XR = XR;
+  P.XT = XT;
+  P.x = x;
+  P.ldx = ldx;
+  P.r = (int)r;
+  P.n = n;
+  P.b = b;
+  P.a = a;
+  P.gl = gl;
+  P.gq = gq;
+  P.GD = GD;
+  P.GL = GL;
+  P.GR = GR;
+  P.GT = GT;
+  P.gy = gy;
+  P.nD = n * b * b;
+  P.nL = (n - 1) * b * b;
+  P.nR = n * a * b;
+  P.nT = a * a;
+  P.nY = gy ? (n * b + a) * r : 0;
+  const int64_t total = P.nD + P.nL + P.nR + P.nT + P.nY;
+  hipStream_t s = (hipStream_t)stream;
+  const int grid = (int)std::min<int64_t>(8 * (int64_t)device_cus(), (total + 255) / 256);
+  if (grid > 0) grad_assemble_kernel<<<grid, 256, 0, s>>>(P);
+  RB_CHECK(hipGetLastError());
+  return ROCBTA_OK;
+}
+
+int rocbta_dquad(int64_t rows, int64_t r, const double* y, const double* x, int64_t ld,
+                 double* out, void* stream) {
+  if (rows < 0 || r < 1 || ld < r || !y || !x || !out) return ROCBTA_ERR_ARG;
+  quad_kernel<<<1, 256, 0, (hipStream_t)stream>>>(y, x, rows, (int)r, ld, out);
+  RB_CHECK(hipGetLastError());
+  return ROCBTA_OK;
 }
 
 int rocbta_dlogdet(int64_t n, int64_t b, int64_t a, const double* D, const double* T,
