@@ -101,6 +101,41 @@ int rocbta_dgrad(int64_t n, int64_t b, int64_t a, const double* XD, const double
 int rocbta_dquad(int64_t rows, int64_t r, const double* y, const double* x, int64_t ld,
                  double* out, void* stream);
 
+/* ---------------------------------------------------------------- batched
+ * Many independent SPD BTA systems (uneven n, b and a <= 64) as one plan,
+ * built once for fixed device arrays and replayed (no counterpart in the
+ * reference; for hyperparameter sweeps that factor many matrices at once).
+ * A run factors every matrix (pobtaf, in one persistent launch), computes
+ * each log-determinant, and with ROCBTA_BATCHED_SOLVE overwrites B[m]
+ * (n*b + a, r[m], row stride r[m]) with A^{-1} B (one persistent launch per
+ * sweep direction), and with ROCBTA_BATCHED_SELINV overwrites the factor
+ * with the selected inverse (pobtasi) as in rocbta_dpobtasi.  Arrays are
+ * overwritten in place exactly like the single-matrix calls. */
+#define ROCBTA_BATCHED_FACTOR 1
+#define ROCBTA_BATCHED_SOLVE 2
+#define ROCBTA_BATCHED_SELINV 4
+typedef void* rocbta_batched_plan;
+
+/* Build a plan (allocates its own device workspace; not stream-ordered). */
+int rocbta_batched_create(int nmat, const int64_t* n, const int64_t* b, const int64_t* a,
+                          double* const* D, double* const* L, double* const* R,
+                          double* const* T, double* const* B, const int64_t* r, int ops,
+                          rocbta_batched_plan* plan);
+/* Workgroup counts of the persistent factorization / solve launches
+ * (<= 0 keeps the current value); used by the autotuner. */
+int rocbta_batched_set_grid(rocbta_batched_plan plan, int factor_grid, int solve_grid);
+/* Enqueue one run on `stream`.  use_graph = 1 captures the launch sequence
+ * into a hipGraph on the first call and replays it afterwards (the stream
+ * must then be capturable, i.e. not the legacy null stream). */
+int rocbta_batched_run(rocbta_batched_plan plan, int use_graph, void* stream);
+/* Synchronise the device and copy back per-matrix info (LAPACK style, -1 =
+ * scheduling time-out), log-determinants, and the solve status (0 or -1);
+ * any pointer may be NULL. */
+int rocbta_batched_results(rocbta_batched_plan plan, int* info, double* logdet, int* status);
+/* Number of kernel launches in one run. */
+int rocbta_batched_launch_count(rocbta_batched_plan plan);
+int rocbta_batched_destroy(rocbta_batched_plan plan);
+
 /* Diagnostic (no counterpart in the reference): runs the task decoding and
  * dependency enumeration of the persistent pobtaf kernel on the HOST for the
  * given shape and returns the number of violations (a task waiting on a task

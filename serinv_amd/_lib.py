@@ -16,6 +16,7 @@ ROCBTA_ERR_ARG = -1
 ROCBTA_ERR_HIP = -2
 ROCBTA_ERR_WORKSPACE = -3
 OP_POBTAF, OP_POBTASI, OP_POBTAS = 0, 1, 2
+BATCHED_FACTOR, BATCHED_SOLVE, BATCHED_SELINV = 1, 2, 4
 
 _ERRORS = {
     ROCBTA_ERR_ARG: "invalid argument",
@@ -43,6 +44,13 @@ SIGNATURES = {
     "rocbta_dgrad": (_c_int, [_c_i64, _c_i64, _c_i64, _c_p, _c_p, _c_p, _c_p, _c_p, _c_i64, _c_i64,
                               _c_p, _c_p, _c_p, _c_p, _c_p, _c_p, _c_p, _c_p]),
     "rocbta_dquad": (_c_int, [_c_i64, _c_i64, _c_p, _c_p, _c_i64, _c_p, _c_p]),
+    "rocbta_batched_create": (_c_int, [_c_int, _c_p, _c_p, _c_p, _c_p, _c_p, _c_p, _c_p, _c_p, _c_p,
+                                       _c_int, ctypes.POINTER(_c_p)]),
+    "rocbta_batched_set_grid": (_c_int, [_c_p, _c_int, _c_int]),
+    "rocbta_batched_run": (_c_int, [_c_p, _c_int, _c_p]),
+    "rocbta_batched_results": (_c_int, [_c_p, _c_p, _c_p, _c_p]),
+    "rocbta_batched_launch_count": (_c_int, [_c_p]),
+    "rocbta_batched_destroy": (_c_int, [_c_p]),
 }
 
 _lib = None

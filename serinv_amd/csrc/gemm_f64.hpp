@@ -254,12 +254,8 @@ __device__ __forceinline__ void gemm_term(const GemmTerm& t, const double* A, co
   }
 }
 
-__global__ void __launch_bounds__(GT_THREADS, 2) gemm_group_kernel(GemmGroup g) {
-  __shared__ __attribute__((aligned(16))) double lds[4 * SLICE];
-  int tile = blockIdx.x;
-  int pi = 0;
-  while (pi + 1 < g.nprob && tile >= g.p[pi + 1].tile_begin) ++pi;
-  const GemmProblem& P = g.p[pi];
+// One 64x64 output tile (global tile index `tile`) of problem P.
+__device__ __forceinline__ void gemm_tile(const GemmProblem& P, int tile, double* lds) {
   int local = (tile - P.tile_begin);
   const int split = local % P.ksplit;
   local /= P.ksplit;
@@ -364,6 +360,28 @@ __global__ void __launch_bounds__(GT_THREADS, 2) gemm_group_kernel(GemmGroup g) 
         *c = v;
         if (P.mirror && col != row) C[(int64_t)col * P.ldc + row] = v;
       }
+}
+
+__global__ void __launch_bounds__(GT_THREADS, 2) gemm_group_kernel(GemmGroup g) {
+  __shared__ __attribute__((aligned(16))) double lds[4 * SLICE];
+  const int tile = blockIdx.x;
+  int pi = 0;
+  while (pi + 1 < g.nprob && tile >= g.p[pi + 1].tile_begin) ++pi;
+  gemm_tile(g.p[pi], tile, lds);
+}
+
+// The same over a device array of problems (sorted by tile_begin): used by
+// the batched plans, whose launch lists are built once and replayed.
+__global__ void __launch_bounds__(GT_THREADS, 2) gemm_list_kernel(const GemmProblem* probs,
+                                                                  int nprob) {
+  __shared__ __attribute__((aligned(16))) double lds[4 * SLICE];
+  const int tile = blockIdx.x;
+  int lo = 0, hi = nprob - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (probs[mid].tile_begin <= tile) lo = mid; else hi = mid - 1;
+  }
+  gemm_tile(probs[lo], tile, lds);
 }
 
 }  // namespace rocbta

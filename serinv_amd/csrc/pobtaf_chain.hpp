@@ -55,6 +55,7 @@ struct PobtafChain {
   int* chain_cu;  // 1 + hardware CU key of the chain workgroup (0 until known)
   double* linv;   // NT * 64 * 64
   double* tipp;   // n * a * a
+  double* tip_inv;  // optional (batched plans): L_T^{-1} as a 64 x 64 tile
   long long spin_limit;
 };
 
@@ -482,6 +483,85 @@ __device__ void chain_run(const PobtafChain& P, double* S, double* Xs, double* W
 }
 
 // ---------------------------------------------------------------- kernel
+// One helper task (already claimed) of matrix P.
+__device__ __forceinline__ void chain_helper_task(const PobtafChain& P, int task, double* lds) {
+  const int tid = threadIdx.x;
+  double* XA = lds;
+  double* XB = lds + CT * KP;
+  int r, j;
+  const int kind = chain_decode(P, task, r, j);
+  if (kind == HK_NOOP) return;  // nobody waits on an empty slot
+
+  f64x4 acc[2][2];
+  if (kind == HK_TIP) {
+    cacc_load({P.T, P.a, P.a, P.a}, acc);
+    for (int i = 0; i < P.n; ++i) {
+      chain_wait(P, P.hflags + t_tippc(P, i), nullptr);
+      f64x4 pi[2][2];
+      cacc_load({P.tipp + (int64_t)i * P.a * P.a, P.a, P.a, P.a}, pi);
+#pragma unroll
+      for (int x = 0; x < 2; ++x)
+#pragma unroll
+        for (int y = 0; y < 2; ++y) acc[x][y] -= pi[x][y];
+    }
+    double (*S2)[TT_LD] = reinterpret_cast<double (*)[TT_LD]>(lds);
+    cacc_to_lds(acc, lds, TT_LD);
+    __syncthreads();
+    lds_potrf(S2, P.a, P.info, (int64_t)P.n * P.b);
+    for (int idx = tid; idx < P.a * P.a; idx += CH_THREADS) {
+      const int rr = idx / P.a, cc = idx % P.a;
+      P.T[idx] = (cc <= rr) ? S2[rr][cc] : 0.0;
+    }
+    if (P.tip_inv) {
+      double (*X2)[TT_LD] = reinterpret_cast<double (*)[TT_LD]>(lds + CT * TT_LD);
+      double (*W2)[WS_LD] = reinterpret_cast<double (*)[WS_LD]>(lds + 2 * CT * TT_LD);
+      __syncthreads();
+      lds_trtri(S2, X2, W2, P.a);
+      for (int idx = tid; idx < CT * CT; idx += CH_THREADS) {
+        const int rr = idx / CT, cc = idx % CT;
+        P.tip_inv[idx] = (rr < P.a && cc <= rr) ? X2[rr][cc] : 0.0;
+      }
+    }
+    chain_publish(P.hflags + task);
+    return;
+  }
+  if (kind == HK_TIPP) {
+    const int i = j;
+    cacc_zero(acc);
+    for (int c = 0; c < P.nt; ++c) {
+      const int k = i * P.nt + c;
+      chain_wait(P, P.hflags + t_arr(P, k), nullptr);
+      TileRegs16 x;
+      tile_load16(tile_arrow(P, k), x);
+      __syncthreads();
+      tile_store16(x, XA);
+      __syncthreads();
+      mfma_abt(XA, XA, acc, 1.0, true);
+    }
+    cacc_store({P.tipp + (int64_t)i * P.a * P.a, P.a, P.a, P.a}, acc);
+    chain_publish(P.hflags + task);
+    return;
+  }
+  CTile out;
+  if (kind == HK_TRSM)
+    out = tile_g(P, r, j);
+  else if (kind == HK_ARR)
+    out = tile_arrow(P, j);
+  else if (kind == HK_PX)
+    out = tile_g(P, j + 1, j);
+  else
+    out = tile_g(P, j, j);  // PD
+  cacc_load(out, acc);
+  const ContribRange cr = chain_contribs(P, kind, r, j);
+  helper_accumulate(P, cr, -1.0, kind == HK_PD, XA, XB, acc);
+  if (kind == HK_TRSM || kind == HK_ARR) helper_solve(P, j, XA, XB, acc);
+  cacc_store(out, acc);
+  // the strict-upper mirror of a lower D-block tile is zero in the factor
+  if (kind == HK_TRSM && r / P.nt == j / P.nt) tile_zero(tile_mirror(P, r, j));
+  if (kind == HK_PX && (j + 1) / P.nt == j / P.nt) tile_zero(tile_mirror(P, j + 1, j));
+  chain_publish(P.hflags + task);
+}
+
 __global__ void __launch_bounds__(CH_THREADS) pobtaf_chain_kernel(PobtafChain P) {
   __shared__ __attribute__((aligned(16))) double lds[2 * CT * KP + CT * WS_LD];
   __shared__ int s_task;
@@ -497,8 +577,6 @@ __global__ void __launch_bounds__(CH_THREADS) pobtaf_chain_kernel(PobtafChain P)
     chain_run(P, lds, lds + CT * TT_LD, lds + 2 * CT * TT_LD);
     return;
   }
-  double* XA = lds;
-  double* XB = lds + CT * KP;
   const int my_cu = 1 + hw_cu_key();
   for (;;) {
     if (tid == 0) {
@@ -509,68 +587,52 @@ __global__ void __launch_bounds__(CH_THREADS) pobtaf_chain_kernel(PobtafChain P)
     const int task = s_task;
     __syncthreads();
     if (task >= P.ntasks) return;
-    int r, j;
-    const int kind = chain_decode(P, task, r, j);
-    if (kind == HK_NOOP) continue;  // nobody waits on an empty slot
+    chain_helper_task(P, task, lds);
+  }
+}
 
-    f64x4 acc[2][2];
-    if (kind == HK_TIP) {
-      cacc_load({P.T, P.a, P.a, P.a}, acc);
-      for (int i = 0; i < P.n; ++i) {
-        chain_wait(P, P.hflags + t_tippc(P, i), nullptr);
-        f64x4 pi[2][2];
-        cacc_load({P.tipp + (int64_t)i * P.a * P.a, P.a, P.a, P.a}, pi);
-#pragma unroll
-        for (int x = 0; x < 2; ++x)
-#pragma unroll
-          for (int y = 0; y < 2; ++y) acc[x][y] -= pi[x][y];
-      }
-      double (*S2)[TT_LD] = reinterpret_cast<double (*)[TT_LD]>(lds);
-      cacc_to_lds(acc, lds, TT_LD);
-      __syncthreads();
-      lds_potrf(S2, P.a, P.info, (int64_t)P.n * P.b);
-      for (int idx = tid; idx < P.a * P.a; idx += CH_THREADS) {
-        const int rr = idx / P.a, cc = idx % P.a;
-        P.T[idx] = (cc <= rr) ? S2[rr][cc] : 0.0;
-      }
-      chain_publish(P.hflags + task);
-      continue;
+// Many independent matrices in one persistent launch (batched plans).  The
+// first nmat claims of *gclaim become the chains of matrices 0..nmat-1;
+// every other workgroup serves the matrices' helper queues round-robin,
+// taking each task from that matrix's own in-order counter, and leaves when
+// a full round finds every queue drained.  cu_busy (4096 ints, zero on
+// entry) marks the CUs that run a chain; helpers there leave at once.
+__global__ void __launch_bounds__(CH_THREADS) pobtaf_chain_multi_kernel(const PobtafChain* Ps,
+                                                                        int nmat, int* gclaim,
+                                                                        int* cu_busy) {
+  __shared__ __attribute__((aligned(16))) double lds[2 * CT * KP + CT * WS_LD];
+  __shared__ int s_task;
+  const int tid = threadIdx.x;
+  if (tid == 0) s_task = atomicAdd(gclaim, 1);
+  __syncthreads();
+  const int first = s_task;
+  __syncthreads();
+  const int my_key = hw_cu_key() & 4095;
+  if (first < nmat) {
+    if (tid == 0) __hip_atomic_store(cu_busy + my_key, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    const PobtafChain& P = Ps[first];
+    chain_run(P, lds, lds + CT * TT_LD, lds + 2 * CT * TT_LD);
+    return;
+  }
+  int m = (int)(blockIdx.x % (unsigned)nmat);
+  int drained = 0;  // consecutive matrices found without tasks
+  while (drained < nmat) {
+    const PobtafChain& P = Ps[m];
+    if (tid == 0) {
+      const int busy = __hip_atomic_load(cu_busy + my_key, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      s_task = busy ? -1 : atomicAdd(P.counter, 1);
     }
-    if (kind == HK_TIPP) {
-      const int i = j;
-      cacc_zero(acc);
-      for (int c = 0; c < P.nt; ++c) {
-        const int k = i * P.nt + c;
-        chain_wait(P, P.hflags + t_arr(P, k), nullptr);
-        TileRegs16 x;
-        tile_load16(tile_arrow(P, k), x);
-        __syncthreads();
-        tile_store16(x, XA);
-        __syncthreads();
-        mfma_abt(XA, XA, acc, 1.0, true);
-      }
-      cacc_store({P.tipp + (int64_t)i * P.a * P.a, P.a, P.a, P.a}, acc);
-      chain_publish(P.hflags + task);
-      continue;
+    __syncthreads();
+    const int task = s_task;
+    __syncthreads();
+    if (task < 0) return;
+    if (task >= P.ntasks) {
+      ++drained;
+    } else {
+      drained = 0;
+      chain_helper_task(P, task, lds);
     }
-    CTile out;
-    if (kind == HK_TRSM)
-      out = tile_g(P, r, j);
-    else if (kind == HK_ARR)
-      out = tile_arrow(P, j);
-    else if (kind == HK_PX)
-      out = tile_g(P, j + 1, j);
-    else
-      out = tile_g(P, j, j);  // PD
-    cacc_load(out, acc);
-    const ContribRange cr = chain_contribs(P, kind, r, j);
-    helper_accumulate(P, cr, -1.0, kind == HK_PD, XA, XB, acc);
-    if (kind == HK_TRSM || kind == HK_ARR) helper_solve(P, j, XA, XB, acc);
-    cacc_store(out, acc);
-    // the strict-upper mirror of a lower D-block tile is zero in the factor
-    if (kind == HK_TRSM && r / P.nt == j / P.nt) tile_zero(tile_mirror(P, r, j));
-    if (kind == HK_PX && (j + 1) / P.nt == j / P.nt) tile_zero(tile_mirror(P, j + 1, j));
-    chain_publish(P.hflags + task);
+    m = (m + 1 == nmat) ? 0 : m + 1;
   }
 }
 

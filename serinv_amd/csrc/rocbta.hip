@@ -92,7 +92,46 @@ void push_term(GemmProblem& p, const GemmTerm& t) {
   p.t[p.nterms++] = t;
 }
 
+// ----------------------------------------------------------- op recording
+// A batched plan records the launches of many independent per-matrix op
+// sequences instead of issuing them: op k of every matrix goes to phase k,
+// and each phase later runs as (at most) one fill, one tile and one GEMM
+// launch over device-resident descriptor lists.
+struct FillOp {
+  double* p;
+  int64_t count;
+  double v;
+};
+struct Phase {
+  std::vector<FillOp> fills;
+  std::vector<TileBatch> tiles;
+  std::vector<GemmProblem> gemms;
+};
+struct Recorder {
+  std::vector<Phase> phases;
+  size_t cur = 0;
+  Phase& next() {
+    if (cur >= phases.size()) phases.emplace_back();
+    return phases[cur++];
+  }
+};
+thread_local Recorder* g_rec = nullptr;
+
 int run_gemms(GemmList& list, hipStream_t s) {
+  if (g_rec) {
+    Phase& ph = g_rec->next();
+    for (GemmProblem p : list.probs) {
+      if (p.M <= 0 || p.N <= 0 || p.batch1 <= 0 || p.batch2 <= 0) continue;
+      if (p.nterms == 0 && p.beta == 1.0) continue;
+      p.tiles_m = cdiv(p.M, GT_M);
+      p.tiles_n = cdiv(p.N, GT_N);
+      p.ksplit = 1;
+      p.partial = nullptr;
+      p.counters = nullptr;
+      ph.gemms.push_back(p);
+    }
+    return ROCBTA_OK;
+  }
   GemmGroup g{};
   auto flush = [&]() -> int {
     if (g.nprob == 0) return ROCBTA_OK;
@@ -128,6 +167,11 @@ __global__ void fill_i32_kernel(int* p, int64_t count, int v) {
 }
 
 int fill(double* p, int64_t count, double v, hipStream_t s) {
+  if (g_rec) {
+    Phase& ph = g_rec->next();
+    if (count > 0) ph.fills.push_back({p, count, v});
+    return ROCBTA_OK;
+  }
   if (count <= 0) return ROCBTA_OK;
   int blocks = (int)std::min<int64_t>(4096, (count + 255) / 256);
   fill_f64_kernel<<<blocks, 256, 0, s>>>(p, count, v);
@@ -140,7 +184,10 @@ int tile_factor(int mode, double* A, int64_t lda, int64_t sA1, int64_t sA2, doub
                 int64_t ldx, int64_t sX1, int64_t sX2, int m, int batch1, int batch2, int* info,
                 int64_t info_row0, int64_t info_s1, int64_t info_s2, int zcols,
                 hipStream_t s) {
-  if (m <= 0 || batch1 <= 0 || batch2 <= 0) return ROCBTA_OK;
+  if (m <= 0 || batch1 <= 0 || batch2 <= 0) {
+    if (g_rec) g_rec->next();
+    return ROCBTA_OK;
+  }
   TileBatch tb{};
   tb.A = A;
   tb.lda = lda;
@@ -158,6 +205,11 @@ int tile_factor(int mode, double* A, int64_t lda, int64_t sA1, int64_t sA2, doub
   tb.info_s1 = info_s1;
   tb.info_s2 = info_s2;
   tb.zcols = zcols;
+  tb.mode = mode;
+  if (g_rec) {
+    g_rec->next().tiles.push_back(tb);
+    return ROCBTA_OK;
+  }
   const int grid = batch1 * batch2;
   if (mode == 0)
     tile_factor_kernel<0><<<grid, TT_THREADS, 0, s>>>(tb);
@@ -1069,3 +1121,5 @@ int rocbta_dlogdet(int64_t n, int64_t b, int64_t a, const double* D, const doubl
 }
 
 }  // extern "C"
+
+#include "batched.hpp"

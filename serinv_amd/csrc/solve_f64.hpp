@@ -67,10 +67,9 @@ __global__ void __launch_bounds__(SV_THREADS)
 
 // out = 2 * (sum_i sum_j log D[i][j][j] + sum_j log T[j][j]); deterministic
 // single-workgroup reduction.
-__global__ void __launch_bounds__(256)
-    logdet_kernel(const double* D, int64_t n, int64_t b, const double* T, int64_t a,
-                  double* out) {
-  __shared__ double red[256];
+__device__ __forceinline__ void logdet_body(const double* D, int64_t n, int64_t b,
+                                            const double* T, int64_t a, double* out,
+                                            double* red) {
   double s = 0.0;
   const int64_t total = n * b;
   for (int64_t idx = threadIdx.x; idx < total; idx += 256) {
@@ -85,6 +84,26 @@ __global__ void __launch_bounds__(256)
     __syncthreads();
   }
   if (threadIdx.x == 0) *out = 2.0 * red[0];
+}
+
+__global__ void __launch_bounds__(256)
+    logdet_kernel(const double* D, int64_t n, int64_t b, const double* T, int64_t a,
+                  double* out) {
+  __shared__ double red[256];
+  logdet_body(D, n, b, T, a, out, red);
+}
+
+// One workgroup per matrix (batched plans).
+struct LogdetDesc {
+  const double* D;
+  const double* T;
+  int64_t n, b, a;
+  double* out;
+};
+__global__ void __launch_bounds__(256) logdet_multi_kernel(const LogdetDesc* d) {
+  __shared__ double red[256];
+  const LogdetDesc& q = d[blockIdx.x];
+  logdet_body(q.D, q.n, q.b, q.T, q.a, q.out, red);
 }
 
 __global__ void fill_f64_kernel(double* p, int64_t count, double v) {

@@ -48,6 +48,7 @@ struct SweepP {
   int r, rc, nchunks;  // columns, chunk width (template RC), chunks
   int trans;           // 0: forward, 1: backward
   const double* tinv;  // NG tiles of 64 x 64 (lower triangle valid)
+  const double* tinv_tip;  // optional: the tip tiles' inverses (else tinv + NT tiles)
   double* ybuf;        // [chunk][NG][64 * RC] published solution tiles
   double* part;        // [chunk][NG][64 * RC] published row partial sums
   double* apart;       // [chunk][n][ta * 64 * RC] published arrow partial sums
@@ -59,6 +60,10 @@ struct SweepP {
 };
 
 // ------------------------------------------------------------ geometry
+__host__ __device__ __forceinline__ const double* sw_tinv(const SweepP& P, int g) {
+  if (g >= P.NT && P.tinv_tip) return P.tinv_tip + (int64_t)(g - P.NT) * SW_T * SW_T;
+  return P.tinv + (int64_t)g * SW_T * SW_T;
+}
 __host__ __device__ __forceinline__ int sw_rows(const SweepP& P, int g) {
   if (g < P.NT) {
     const int q = g % P.nt;
@@ -368,7 +373,7 @@ __device__ void sw_tip_step(const SweepP& P, int chunk, int t, int cols, SwLds<R
   }
   sw_reduce<RC>(acc, S.red, S.V1);  // V1 = reduced right-hand side
   SwRegs x;
-  sw_load<true>(P.tinv + (int64_t)g * SW_T * SW_T, SW_T, m, m, x);
+  sw_load<true>(sw_tinv(P, g), SW_T, m, m, x);
   sw_store_lds(x, S.A);
   __syncthreads();
 #pragma unroll
@@ -408,7 +413,7 @@ __device__ void sw_chain(const SweepP& P, int chunk, SwLds<RC>& S) {
       else
         sw_load<false>(sw_tile(P, gp, g, ld), ld, sw_rows(P, gp), m, rs);
     }
-    sw_load<true>(P.tinv + (int64_t)g * SW_T * SW_T, SW_T, m, m, rt);
+    sw_load<true>(sw_tinv(P, g), SW_T, m, m, rt);
     sw_vload<RC>(P, sw_bptr<RC>(P, chunk, g), P.ldb, m, cols, rb);
     if (sw_row_nonempty(P, g)) sw_take<RC>(P, sw_blk<RC>(P.part, P, chunk, g), S.V2);
   };
@@ -542,6 +547,47 @@ __global__ void __launch_bounds__(SW_THREADS) pobtas_sweep_kernel(SweepP P) {
     const int kind = sw_decode(P, task, chunk, g);
     if (kind == SK_NOOP) continue;
     sw_helper<RC>(P, kind, chunk, g, S);
+  }
+}
+
+// Many independent systems in one persistent launch (batched plans): claims
+// below chain_begin[nmat] become the chains of (matrix, chunk) pairs
+// (chain_begin: prefix sums of the matrices' chunk counts); every other
+// workgroup serves the matrices' helper queues round-robin from each
+// matrix's own in-order counter and leaves when a full round finds every
+// queue drained.
+template <int RC>
+__global__ void __launch_bounds__(SW_THREADS)
+    pobtas_sweep_multi_kernel(const SweepP* Ps, const int* chain_begin, int nmat, int* gclaim) {
+  __shared__ __attribute__((aligned(16))) SwLds<RC> S;
+  __shared__ int s_task;
+  if (threadIdx.x == 0) s_task = atomicAdd(gclaim, 1);
+  __syncthreads();
+  const int first = s_task;
+  __syncthreads();
+  if (first < chain_begin[nmat]) {
+    int m = 0;
+    while (chain_begin[m + 1] <= first) ++m;
+    sw_chain<RC>(Ps[m], first - chain_begin[m], S);
+    return;
+  }
+  int m = (int)(blockIdx.x % (unsigned)nmat);
+  int drained = 0;
+  while (drained < nmat) {
+    const SweepP& P = Ps[m];
+    if (threadIdx.x == 0) s_task = atomicAdd(P.counter, 1);
+    __syncthreads();
+    const int task = s_task;
+    __syncthreads();
+    if (task >= P.ntasks) {
+      ++drained;
+    } else {
+      drained = 0;
+      int chunk, g;
+      const int kind = sw_decode(P, task, chunk, g);
+      if (kind != SK_NOOP) sw_helper<RC>(P, kind, chunk, g, S);
+    }
+    m = (m + 1 == nmat) ? 0 : m + 1;
   }
 }
 

@@ -36,6 +36,8 @@ struct TileBatch {
   // Strict-upper zero fill of the rows of the tile that lie right of it:
   // zcols columns starting at column m of each tile row (0 = none).
   int zcols;
+  int mode;         // list launches: 0 potrf (+ inverse), 1 inverse only
+  int grid_begin;   // list launches: first workgroup of this entry
 };
 
 __device__ __forceinline__ double readlane_f64(double v, int lane) {
@@ -260,11 +262,8 @@ __device__ __forceinline__ void lds_trtri(const double (*S)[TT_LD], double (*Xs)
 
 // mode 0: potrf (+ inverse into X if X != nullptr); mode 1: trtri only.
 template <int MODE>
-__global__ void __launch_bounds__(TT_THREADS) tile_factor_kernel(TileBatch tb) {
-  __shared__ double S[TT][TT_LD];
-  __shared__ double Xs[TT][TT_LD];
-  __shared__ double Ws[TT][WS_LD];
-  const int bz = blockIdx.x;
+__device__ __forceinline__ void tile_factor_body(const TileBatch& tb, int bz, double (*S)[TT_LD],
+                                                 double (*Xs)[TT_LD], double (*Ws)[WS_LD]) {
   const int b1 = bz % tb.batch1, b2 = bz / tb.batch1;
   double* A = tb.A + b1 * tb.sA1 + b2 * tb.sA2;
   const int m = tb.m;
@@ -296,6 +295,34 @@ __global__ void __launch_bounds__(TT_THREADS) tile_factor_kernel(TileBatch tb) {
       X[(int64_t)r * tb.ldx + c] = Xs[r][c];
     }
   }
+}
+
+template <int MODE>
+__global__ void __launch_bounds__(TT_THREADS) tile_factor_kernel(TileBatch tb) {
+  __shared__ double S[TT][TT_LD];
+  __shared__ double Xs[TT][TT_LD];
+  __shared__ double Ws[TT][WS_LD];
+  tile_factor_body<MODE>(tb, blockIdx.x, S, Xs, Ws);
+}
+
+// The same over a device array of entries sorted by grid_begin (batched
+// plans); each entry runs in its own mode.
+__global__ void __launch_bounds__(TT_THREADS) tile_factor_list_kernel(const TileBatch* tbs,
+                                                                      int ntb) {
+  __shared__ double S[TT][TT_LD];
+  __shared__ double Xs[TT][TT_LD];
+  __shared__ double Ws[TT][WS_LD];
+  const int wg = blockIdx.x;
+  int lo = 0, hi = ntb - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (tbs[mid].grid_begin <= wg) lo = mid; else hi = mid - 1;
+  }
+  const TileBatch& tb = tbs[lo];
+  if (tb.mode == 0)
+    tile_factor_body<0>(tb, wg - tb.grid_begin, S, Xs, Ws);
+  else
+    tile_factor_body<1>(tb, wg - tb.grid_begin, S, Xs, Ws);
 }
 
 }  // namespace rocbta
