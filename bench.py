@@ -42,6 +42,7 @@ def parse():
     p.add_argument("--no-cpu-baseline", action="store_true")
     p.add_argument("--cpu-seconds", type=float, default=15.0)
     p.add_argument("--no-solve", action="store_true")
+    p.add_argument("--no-batched", action="store_true")
     return p.parse_args()
 
 
@@ -112,6 +113,97 @@ def cpu_baseline(b, a, n_full, budget_s):
                   f"({taf + tsi:.2f} s measured: pobtaf {taf:.2f} s, pobtasi {tsi:.2f} s), "
                   f"scaled by FLOP ratio {f_full / f_s:.2f}; BLAS {blas}",
         "tflops": round(f_s / (taf + tsi) * 1e-12, 4),
+    }
+
+
+def bench_batched(dev, nmat=64, r=4, reps=3, tune=True):
+    """64 independent SPD BTA systems (n ~ U{16..256}, b in {64, 128, 256},
+    a ~ U{4..32}, seed 63 + k, 4 RHS): factor + solve + selected inversion,
+    as one replayed batched plan (hipGraph) against a loop of the plain
+    per-matrix calls.  Returns the timings and the largest difference
+    between the two paths' outputs."""
+    import numpy as np
+    import torch
+
+    import serinv_amd.algs as algs
+    from serinv_amd.autotune import tune_batched
+    from serinv_amd.batched import BatchedBTA
+    from serinv_amd.utils.matrices import bta_flops, bta_solve_flops, bta_spd
+
+    rng = np.random.default_rng(63)
+    shapes = [(int(rng.integers(16, 257)), int(rng.choice([64, 128, 256])), int(rng.integers(4, 33)))
+              for _ in range(nmat)]
+    orig, work = [], []
+    for k, (n, b, a) in enumerate(shapes):
+        arrs = bta_spd(n, b, a, seed=63 + k, n_rhs=r)
+        o = [torch.from_numpy(x).to(dev) for x in arrs]
+        orig.append(o)
+        work.append([t.clone() for t in o])
+    flops = sum(sum(bta_flops(n, b, a)) + 2 * bta_solve_flops(n, b, a, r) for n, b, a in shapes)
+
+    def restore():
+        for w, o in zip(work, orig):
+            for t, s in zip(w, o):
+                t.copy_(s)
+        torch.cuda.synchronize(dev)
+
+    stream = torch.cuda.current_stream(dev)
+    algs.config["check_info"] = False
+    loop = []
+    for _ in range(reps):
+        restore()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        for D, L, R, T, B in work:
+            algs.pobtaf(D, L, R, T)
+            algs.pobtas(D, L, R, T, B, trans="N")
+            algs.pobtas(D, L, R, T, B, trans="C")
+            algs.pobtasi(D, L, R, T)
+        e1.record(stream)
+        torch.cuda.synchronize(dev)
+        loop.append(e0.elapsed_time(e1))
+    loop_out = [[t.clone() for t in w] for w in work]
+    tuned = None
+    if tune:
+        tuned = tune_batched([w[:4] for w in work], rhs=[w[4] for w in work], selinv=True,
+                             use_cache=False)
+    plan = BatchedBTA([w[:4] for w in work], rhs=[w[4] for w in work], selinv=True, graph=True)
+    if tuned:
+        plan.set_grid(tuned["factor_grid"], tuned["solve_grid"])
+    restore()
+    plan.run()  # capture + first replay
+    torch.cuda.synchronize(dev)
+    bat = []
+    for _ in range(reps):
+        restore()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        plan.run()
+        e1.record(stream)
+        torch.cuda.synchronize(dev)
+        bat.append(e0.elapsed_time(e1))
+    info, _, status = plan.results()
+    diff = 0.0
+    for w, lo in zip(work, loop_out):
+        for t, s in zip(w, lo):
+            if t.numel():
+                diff = max(diff, float((t - s).abs().max() / s.abs().max().clamp_min(1e-300)))
+    launches = plan.launch_count()
+    plan.close()
+    algs.config["check_info"] = True
+    t_loop, t_bat = min(loop), min(bat)
+    return {
+        "workload": f"{nmat} SPD BTA systems, n~U[16,256], b in {{64,128,256}}, a~U[4,32], r={r}; "
+                    "pobtaf + pobtas(N,C) + pobtasi, FP64",
+        "batched_ms": round(t_bat, 3),
+        "loop_ms": round(t_loop, 3),
+        "speedup": round(t_loop / t_bat, 2),
+        "matrices_per_s": round(nmat / (t_bat * 1e-3), 1),
+        "tflops": round(flops / (t_bat * 1e-3) * 1e-12, 3),
+        "launches_per_run": launches,
+        "max_rel_diff_vs_loop": float(f"{diff:.3e}"),
+        "info_ok": bool(np.all(info == 0) and status == 0),
+        "autotune": tuned,
     }
 
 
@@ -257,6 +349,12 @@ def main():
         "roofline": roofline,
         **extra,
     }
+    if not args.no_batched:
+        print("[bench] batched plan vs per-matrix loop ...", file=sys.stderr, flush=True)
+        out["batched"] = bench_batched(dev)
+        out["batched_speedup"] = out["batched"]["speedup"]
+        print(f"[bench] batched {out['batched']['batched_ms']} ms vs loop {out['batched']['loop_ms']} ms",
+              file=sys.stderr, flush=True)
     if not args.no_cpu_baseline:
         print("[bench] timing the CPU baseline ...", file=sys.stderr, flush=True)
         out["cpu_baseline"] = cpu_baseline(b, a, n, args.cpu_seconds)

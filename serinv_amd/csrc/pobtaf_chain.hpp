@@ -363,6 +363,10 @@ __device__ __forceinline__ void tile_zero(const CTile& t) {
 // ------------------------------------------------------------ helper work
 // acc -= sum over the contribution range (A = factor tile (ra,k) or arrow
 // tile k, B = factor tile (jb,k)); operands prefetched one product ahead.
+// TAG gives each kernel its own copy of the non-inlined workers, so that
+// each copy sees a single kind of descriptor pointer (kernel argument or
+// global array) and is optimised for it.
+template <int TAG>
 __device__ void helper_accumulate(const PobtafChain& P, const ContribRange& cr, double sgn,
                                   bool lower_only, double* XA, double* XB, f64x4 (&acc)[2][2]) {
   if (cr.k0 >= cr.k1) return;
@@ -389,6 +393,7 @@ __device__ void helper_accumulate(const PobtafChain& P, const ContribRange& cr, 
 }
 
 // out = acc * Linv(j)^T
+template <int TAG>
 __device__ void helper_solve(const PobtafChain& P, int j, double* XA, double* XB,
                              f64x4 (&acc)[2][2]) {
   chain_wait(P, P.cflags + 2 * j, nullptr);
@@ -405,6 +410,7 @@ __device__ void helper_solve(const PobtafChain& P, int j, double* XA, double* XB
 // LDS plan of the chain workgroup: S [64][65] (potrf, then L(g+1,g) staged
 // K-contiguous with pitch 65 for the next step's update), Xs [64][65]
 // (Linv(g)), Ws [64][17] (trtri scratch).
+template <int TAG>
 __device__ void chain_run(const PobtafChain& P, double* S, double* Xs, double* Ws) {
   const int tid = threadIdx.x;
   double (*S2)[TT_LD] = reinterpret_cast<double (*)[TT_LD]>(S);
@@ -484,6 +490,7 @@ __device__ void chain_run(const PobtafChain& P, double* S, double* Xs, double* W
 
 // ---------------------------------------------------------------- kernel
 // One helper task (already claimed) of matrix P.
+template <int TAG>
 __device__ __forceinline__ void chain_helper_task(const PobtafChain& P, int task, double* lds) {
   const int tid = threadIdx.x;
   double* XA = lds;
@@ -553,8 +560,8 @@ __device__ __forceinline__ void chain_helper_task(const PobtafChain& P, int task
     out = tile_g(P, j, j);  // PD
   cacc_load(out, acc);
   const ContribRange cr = chain_contribs(P, kind, r, j);
-  helper_accumulate(P, cr, -1.0, kind == HK_PD, XA, XB, acc);
-  if (kind == HK_TRSM || kind == HK_ARR) helper_solve(P, j, XA, XB, acc);
+  helper_accumulate<TAG>(P, cr, -1.0, kind == HK_PD, XA, XB, acc);
+  if (kind == HK_TRSM || kind == HK_ARR) helper_solve<TAG>(P, j, XA, XB, acc);
   cacc_store(out, acc);
   // the strict-upper mirror of a lower D-block tile is zero in the factor
   if (kind == HK_TRSM && r / P.nt == j / P.nt) tile_zero(tile_mirror(P, r, j));
@@ -574,7 +581,7 @@ __global__ void __launch_bounds__(CH_THREADS) pobtaf_chain_kernel(PobtafChain P)
     // the chain runs latency-bound code: keep helpers off its CU
     if (tid == 0)
       __hip_atomic_store(P.chain_cu, 1 + hw_cu_key(), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    chain_run(P, lds, lds + CT * TT_LD, lds + 2 * CT * TT_LD);
+    chain_run<0>(P, lds, lds + CT * TT_LD, lds + 2 * CT * TT_LD);
     return;
   }
   const int my_cu = 1 + hw_cu_key();
@@ -587,7 +594,7 @@ __global__ void __launch_bounds__(CH_THREADS) pobtaf_chain_kernel(PobtafChain P)
     const int task = s_task;
     __syncthreads();
     if (task >= P.ntasks) return;
-    chain_helper_task(P, task, lds);
+    chain_helper_task<0>(P, task, lds);
   }
 }
 
@@ -611,7 +618,7 @@ __global__ void __launch_bounds__(CH_THREADS) pobtaf_chain_multi_kernel(const Po
   if (first < nmat) {
     if (tid == 0) __hip_atomic_store(cu_busy + my_key, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     const PobtafChain& P = Ps[first];
-    chain_run(P, lds, lds + CT * TT_LD, lds + 2 * CT * TT_LD);
+    chain_run<1>(P, lds, lds + CT * TT_LD, lds + 2 * CT * TT_LD);
     return;
   }
   int m = (int)(blockIdx.x % (unsigned)nmat);
@@ -630,7 +637,7 @@ __global__ void __launch_bounds__(CH_THREADS) pobtaf_chain_multi_kernel(const Po
       ++drained;
     } else {
       drained = 0;
-      chain_helper_task(P, task, lds);
+      chain_helper_task<1>(P, task, lds);
     }
     m = (m + 1 == nmat) ? 0 : m + 1;
   }
