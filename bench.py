@@ -51,7 +51,7 @@ def calibrated_gamma(n, b, a):
     try:
         with open(path) as f:
             table = json.load(f)
-        return float(table[f"{n},{b},{a},1e6"]["gamma"]), "kappa~1e6 (calibrated)"
+        return float(table[f"{n},{b},{a},{1e6:.0e}"]["gamma"]), "kappa~1e6 (calibrated)"
     except (OSError, KeyError, ValueError):
         return 1.0, "gamma=1 (strictly diagonally dominant)"
 

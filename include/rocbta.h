@@ -38,6 +38,7 @@ extern "C" {
 #define ROCBTA_OP_POBTAF 0
 #define ROCBTA_OP_POBTASI 1
 #define ROCBTA_OP_POBTAS 2
+#define ROCBTA_OP_RESIDUAL 3
 
 /* Library version string. */
 const char* rocbta_version(void);
@@ -80,6 +81,30 @@ int rocbta_dpobtas(int64_t n, int64_t b, int64_t a, const double* D, const doubl
  * No counterpart in the reference (extension for likelihood evaluation). */
 int rocbta_dlogdet(int64_t n, int64_t b, int64_t a, const double* D, const double* T,
                    double* out, void* stream);
+
+/* Mixed-precision factorization / selected inversion (no counterpart in the
+ * reference).  Same arguments, layout, workspace and results as
+ * rocbta_dpobtaf / rocbta_dpobtasi; the trsm, syrk and gemm tile products
+ * round their FP64 operands to FP32 and accumulate in FP32 MFMA, while the
+ * blocks stay FP64 in memory and the diagonal-tile potrf / trtri, the
+ * arrow-tip factorization and rocbta_dlogdet run in FP64.  Accuracy is that
+ * of FP32 products (relative error ~ cond(A) * 1e-7); solves regain FP64
+ * accuracy by iterative refinement with rocbta_dresidual. */
+int rocbta_mpobtaf(int64_t n, int64_t b, int64_t a, double* D, double* L, double* R, double* T,
+                   int factorize_last_block, int* info, void* workspace, size_t workspace_bytes,
+                   void* stream);
+int rocbta_mpobtasi(int64_t n, int64_t b, int64_t a, double* D, double* L, double* R, double* T,
+                    int invert_last_block, void* workspace, size_t workspace_bytes, void* stream);
+
+/* FP64 residual Out = B - A X for the SPD BTA matrix A given by its stored
+ * blocks (lower triangles of D and T are used; L = A_{i+1,i}, R = A_{t,i}).
+ * X, B, Out: (n*b + a, r) with row stride ld; Out may alias B, not X.
+ * Workspace: rocbta_workspace_bytes(ROCBTA_OP_RESIDUAL, n, b, a, r).  No
+ * counterpart in the reference (iterative refinement of mixed solves). */
+int rocbta_dresidual(int64_t n, int64_t b, int64_t a, const double* D, const double* L,
+                     const double* R, const double* T, const double* X, const double* B,
+                     double* Out, int64_t r, int64_t ld, void* workspace, size_t workspace_bytes,
+                     void* stream);
 
 /* Gradient of f = gl * log det A + gq * sum_k y_k^T A^{-1} y_k with respect
  * to the stored blocks of A (no counterpart in the reference; the backward of

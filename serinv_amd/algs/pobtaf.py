@@ -34,12 +34,13 @@ def pobtaf(A_diagonal_blocks, A_lower_diagonal_blocks, A_lower_arrow_blocks, A_a
     _factor(D, L, R, T, n, b, a, dev, factorize_last_block, "pobtaf")
 
 
-def _factor(D, L, R, T, n, b, a, dev, factorize_last_block, what):
+def _factor(D, L, R, T, n, b, a, dev, factorize_last_block, what, mixed=False):
     lib = _lib.load()
     ws, nbytes = workspace(_lib.OP_POBTAF, n, b, a, 0, dev)
     info = torch.empty(1, dtype=torch.int32, device=dev)
-    rc = lib.rocbta_dpobtaf(n, b, a, ptr(D), ptr(L), ptr(R), ptr(T), int(factorize_last_block),
-                            info.data_ptr(), ws.data_ptr(), nbytes, stream_of(dev))
+    fn = lib.rocbta_mpobtaf if mixed else lib.rocbta_dpobtaf
+    rc = fn(n, b, a, ptr(D), ptr(L), ptr(R), ptr(T), int(factorize_last_block),
+            info.data_ptr(), ws.data_ptr(), nbytes, stream_of(dev))
     _lib.check(rc, what)
     if config["check_info"]:
         raise_on_info(info, what)

@@ -25,9 +25,10 @@ def pobtasi(L_diagonal_blocks, L_lower_diagonal_blocks, L_lower_arrow_blocks, L_
     _selinv(D, L, R, T, n, b, a, dev, invert_last_block, "pobtasi")
 
 
-def _selinv(D, L, R, T, n, b, a, dev, invert_last_block, what):
+def _selinv(D, L, R, T, n, b, a, dev, invert_last_block, what, mixed=False):
     lib = _lib.load()
     ws, nbytes = workspace(_lib.OP_POBTASI, n, b, a, 0, dev)
-    rc = lib.rocbta_dpobtasi(n, b, a, ptr(D), ptr(L), ptr(R), ptr(T), int(invert_last_block),
-                             ws.data_ptr(), nbytes, stream_of(dev))
+    fn = lib.rocbta_mpobtasi if mixed else lib.rocbta_dpobtasi
+    rc = fn(n, b, a, ptr(D), ptr(L), ptr(R), ptr(T), int(invert_last_block),
+            ws.data_ptr(), nbytes, stream_of(dev))
     _lib.check(rc, what)

@@ -32,6 +32,15 @@ __device__ __forceinline__ int acc_row_f64(int lane, int r) { return 4 * r + (la
 __device__ __forceinline__ int acc_row_f32(int lane, int r) { return 4 * (lane >> 4) + r; }
 __device__ __forceinline__ int acc_col(int lane) { return lane & 15; }
 
+// FP64 operands into either accumulator: f64x4 runs FP64 MFMA, f32x4 rounds
+// the operands to FP32 and runs FP32 MFMA (the mixed-precision mode).
+__device__ __forceinline__ f64x4 mfma_acc(double a, double b, f64x4 c) { return mfma_f64(a, b, c); }
+__device__ __forceinline__ f32x4 mfma_acc(double a, double b, f32x4 c) {
+  return mfma_f32((float)a, (float)b, c);
+}
+__device__ __forceinline__ int acc_row(const f64x4&, int lane, int r) { return acc_row_f64(lane, r); }
+__device__ __forceinline__ int acc_row(const f32x4&, int lane, int r) { return acc_row_f32(lane, r); }
+
 // Record an LAPACK-style failure index (1-based; smallest index wins; 0 = ok).
 __device__ __forceinline__ void record_info(int* info, int value) {
   if (!info) return;

@@ -57,6 +57,10 @@ struct GemmProblem {
   int ksplit;
   double* partial;
   int* counters;
+  // Mixed precision: operands are read as FP64 and rounded to FP32 at the
+  // MFMA, products accumulate in FP32 (v_mfma_f32_16x16x4_f32); the epilogue
+  // (beta * C + result) runs in FP64.
+  int f32;
   GemmTerm t[2];
 };
 
@@ -177,9 +181,9 @@ struct TermView {
 };
 
 // One LDS stage of MFMAs (wave quadrant 32x32 = 2x2 MFMA tiles).
-template <bool TA, bool TB>
+template <bool TA, bool TB, typename ACC>
 __device__ __forceinline__ void gemm_mfma_stage(const double* XA, const double* XB,
-                                                f64x4 (&acc)[2][2]) {
+                                                ACC (&acc)[2][2]) {
   const int lane = threadIdx.x & 63;
   const int wave = threadIdx.x >> 6;
   const int wm = (wave >> 1) * 32, wn = (wave & 1) * 32;
@@ -190,19 +194,19 @@ __device__ __forceinline__ void gemm_mfma_stage(const double* XA, const double* 
     const double a1 = slice_at<!TA>(XA, wm + 16 + li, kk + lk);
     const double b0 = slice_at<TB>(XB, wn + li, kk + lk);
     const double b1 = slice_at<TB>(XB, wn + 16 + li, kk + lk);
-    acc[0][0] = mfma_f64(a0, b0, acc[0][0]);
-    acc[0][1] = mfma_f64(a0, b1, acc[0][1]);
-    acc[1][0] = mfma_f64(a1, b0, acc[1][0]);
-    acc[1][1] = mfma_f64(a1, b1, acc[1][1]);
+    acc[0][0] = mfma_acc(a0, b0, acc[0][0]);
+    acc[0][1] = mfma_acc(a0, b1, acc[0][1]);
+    acc[1][0] = mfma_acc(a1, b0, acc[1][0]);
+    acc[1][1] = mfma_acc(a1, b1, acc[1][1]);
   }
 }
 
 // acc += alpha * op(A) op(B) over the K range [kb, ke): LDS double buffer
 // (slots 0/1) and two register stages in flight (x/y), unrolled by two so
 // that every register set has a fixed role.
-template <bool TA, bool TB, bool FAST>
+template <bool TA, bool TB, bool FAST, typename ACC>
 __device__ __forceinline__ void gemm_term_pipeline(const TermView<TA, TB, FAST>& v, int kb,
-                                                   double* lds, f64x4 (&acc)[2][2]) {
+                                                   double* lds, ACC (&acc)[2][2]) {
   const int ke = v.ke;
   const int nst = kb < ke ? (ke - kb + GT_K - 1) / GT_K : 0;
   if (nst == 0) return;
@@ -219,12 +223,12 @@ __device__ __forceinline__ void gemm_term_pipeline(const TermView<TA, TB, FAST>&
   if (nst > 2) v.load(kb + 2 * GT_K, xa, xb);
   for (int s = 0; s < nst; s += 2) {
     // slot 0 holds stage s; y holds stage s+1; x holds stage s+2 (in flight)
-    gemm_mfma_stage<TA, TB>(A0, B0, acc);
+    gemm_mfma_stage<TA, TB, ACC>(A0, B0, acc);
     if (s + 1 >= nst) break;
     v.store(ya, yb, A1, B1);
     __syncthreads();
     if (s + 3 < nst) v.load(kb + (s + 3) * GT_K, ya, yb);
-    gemm_mfma_stage<TA, TB>(A1, B1, acc);
+    gemm_mfma_stage<TA, TB, ACC>(A1, B1, acc);
     if (s + 2 >= nst) break;
     v.store(xa, xb, A0, B0);
     __syncthreads();
@@ -234,10 +238,10 @@ __device__ __forceinline__ void gemm_term_pipeline(const TermView<TA, TB, FAST>&
 
 // Stages [sb, se) of the term (a stage = GT_K consecutive k from the term's
 // first k); the K range of those stages is clipped to the term's range.
-template <bool TA, bool TB>
+template <bool TA, bool TB, typename ACC>
 __device__ __forceinline__ void gemm_term(const GemmTerm& t, const double* A, const double* B,
                                           int M, int N, int m0, int n0, int sb, int se,
-                                          double* lds, f64x4 (&acc)[2][2]) {
+                                          double* lds, ACC (&acc)[2][2]) {
   int kb, ke;
   gemm_k_range(t, m0, n0, kb, ke);
   ke = min(ke, kb + se * GT_K);
@@ -247,15 +251,17 @@ __device__ __forceinline__ void gemm_term(const GemmTerm& t, const double* A, co
                     n0 + GT_N <= N && ((ke - kb) % GT_K) == 0;
   if (fast) {
     TermView<TA, TB, true> v{A, B, t.lda, t.ldb, M, N, m0, n0, ke, t.alpha};
-    gemm_term_pipeline<TA, TB, true>(v, kb, lds, acc);
+    gemm_term_pipeline<TA, TB, true, ACC>(v, kb, lds, acc);
   } else {
     TermView<TA, TB, false> v{A, B, t.lda, t.ldb, M, N, m0, n0, ke, t.alpha};
-    gemm_term_pipeline<TA, TB, false>(v, kb, lds, acc);
+    gemm_term_pipeline<TA, TB, false, ACC>(v, kb, lds, acc);
   }
 }
 
-// One 64x64 output tile (global tile index `tile`) of problem P.
-__device__ __forceinline__ void gemm_tile(const GemmProblem& P, int tile, double* lds) {
+// One 64x64 output tile (global tile index `tile`) of problem P, with
+// accumulators of type ACC (f64x4: FP64, f32x4: mixed precision).
+template <typename ACC>
+__device__ __forceinline__ void gemm_tile_acc(const GemmProblem& P, int tile, double* lds) {
   int local = (tile - P.tile_begin);
   const int split = local % P.ksplit;
   local /= P.ksplit;
@@ -274,11 +280,11 @@ __device__ __forceinline__ void gemm_tile(const GemmProblem& P, int tile, double
   const int wave = threadIdx.x >> 6;
   const int wm = (wave >> 1) * 32, wn = (wave & 1) * 32;
 
-  f64x4 acc[2][2];
+  ACC acc[2][2];
 #pragma unroll
   for (int i = 0; i < 2; ++i)
 #pragma unroll
-    for (int j = 0; j < 2; ++j) acc[i][j] = f64x4{0.0, 0.0, 0.0, 0.0};
+    for (int j = 0; j < 2; ++j) acc[i][j] = ACC{0, 0, 0, 0};
 
   // stage counts of the terms; this workgroup's share of the stage sequence
   int nst[2] = {0, 0};
@@ -298,13 +304,13 @@ __device__ __forceinline__ void gemm_tile(const GemmProblem& P, int tile, double
     const double* A = t.A + b1 * t.sA1 + b2 * t.sA2;
     const double* B = t.B + b1 * t.sB1 + b2 * t.sB2;
     if (!t.transA && !t.transB)
-      gemm_term<false, false>(t, A, B, P.M, P.N, m0, n0, sb, se, lds, acc);
+      gemm_term<false, false, ACC>(t, A, B, P.M, P.N, m0, n0, sb, se, lds, acc);
     else if (!t.transA && t.transB)
-      gemm_term<false, true>(t, A, B, P.M, P.N, m0, n0, sb, se, lds, acc);
+      gemm_term<false, true, ACC>(t, A, B, P.M, P.N, m0, n0, sb, se, lds, acc);
     else if (t.transA && !t.transB)
-      gemm_term<true, false>(t, A, B, P.M, P.N, m0, n0, sb, se, lds, acc);
+      gemm_term<true, false, ACC>(t, A, B, P.M, P.N, m0, n0, sb, se, lds, acc);
     else
-      gemm_term<true, true>(t, A, B, P.M, P.N, m0, n0, sb, se, lds, acc);
+      gemm_term<true, true, ACC>(t, A, B, P.M, P.N, m0, n0, sb, se, lds, acc);
   }
 
   if (P.ksplit > 1) {
@@ -339,7 +345,7 @@ __device__ __forceinline__ void gemm_tile(const GemmProblem& P, int tile, double
           const int idx = ((i * 2 + j) * 4 + r) * GT_THREADS + tid;
           double sum = 0.0;
           for (int sp = 0; sp < P.ksplit; ++sp)
-            sum += (sp == split) ? acc[i][j][r] : base[(int64_t)sp * (GT_M * GT_N) + idx];
+            sum += (sp == split) ? (double)acc[i][j][r] : base[(int64_t)sp * (GT_M * GT_N) + idx];
           acc[i][j][r] = sum;
         }
   }
@@ -350,16 +356,24 @@ __device__ __forceinline__ void gemm_tile(const GemmProblem& P, int tile, double
     for (int j = 0; j < 2; ++j)
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
-        const int row = m0 + wm + i * 16 + acc_row_f64(lane, r);
+        const int row = m0 + wm + i * 16 + acc_row(acc[i][j], lane, r);
         const int col = n0 + wn + j * 16 + acc_col(lane);
         if (row >= P.M || col >= P.N) continue;
         if (P.uplo == TRI_LOWER && col > row) continue;
         if (P.uplo == TRI_UPPER && col < row) continue;
         double* c = C + (int64_t)row * P.ldc + col;
-        const double v = (P.beta != 0.0) ? acc[i][j][r] + P.beta * (*c) : acc[i][j][r];
+        const double av = (double)acc[i][j][r];
+        const double v = (P.beta != 0.0) ? av + P.beta * (*c) : av;
         *c = v;
         if (P.mirror && col != row) C[(int64_t)col * P.ldc + row] = v;
       }
+}
+
+__device__ __forceinline__ void gemm_tile(const GemmProblem& P, int tile, double* lds) {
+  if (P.f32)
+    gemm_tile_acc<f32x4>(P, tile, lds);
+  else
+    gemm_tile_acc<f64x4>(P, tile, lds);
 }
 
 __global__ void __launch_bounds__(GT_THREADS, 2) gemm_group_kernel(GemmGroup g) {

@@ -57,6 +57,10 @@ struct PobtafChain {
   double* tipp;   // n * a * a
   double* tip_inv;  // optional (batched plans): L_T^{-1} as a 64 x 64 tile
   long long spin_limit;
+  // Mixed precision: the tile products (trsm, syrk/gemm updates) round their
+  // FP64 operands to FP32 and accumulate in FP32 MFMA; the tiles stay FP64
+  // in memory and the diagonal potrf / trtri and the tip run in FP64.
+  int f32;
 };
 
 // Hardware key of the CU running the calling wave: XCC id and the
@@ -284,7 +288,8 @@ __device__ __forceinline__ void tile_store16(const TileRegs16& x, double* X) {
 }
 
 // acc (+)= sgn * A B^T over k in [0, 64), A = XA[m][k], B = XB[n][k].
-__device__ __forceinline__ void mfma_abt(const double* XA, const double* XB, f64x4 (&acc)[2][2],
+template <typename ACC>
+__device__ __forceinline__ void mfma_abt(const double* XA, const double* XB, ACC (&acc)[2][2],
                                          double sgn, bool lower_only) {
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int wm = (wave >> 1) * 32, wn = (wave & 1) * 32;
@@ -296,11 +301,45 @@ __device__ __forceinline__ void mfma_abt(const double* XA, const double* XB, f64
     const double a1 = sgn * XA[(wm + 16 + li) * KP + kk + lk];
     const double b0 = XB[(wn + li) * KP + kk + lk];
     const double b1 = XB[(wn + 16 + li) * KP + kk + lk];
-    acc[0][0] = mfma_f64(a0, b0, acc[0][0]);
-    acc[0][1] = mfma_f64(a0, b1, acc[0][1]);
-    acc[1][0] = mfma_f64(a1, b0, acc[1][0]);
-    acc[1][1] = mfma_f64(a1, b1, acc[1][1]);
+    acc[0][0] = mfma_acc(a0, b0, acc[0][0]);
+    acc[0][1] = mfma_acc(a0, b1, acc[0][1]);
+    acc[1][0] = mfma_acc(a1, b0, acc[1][0]);
+    acc[1][1] = mfma_acc(a1, b1, acc[1][1]);
   }
+}
+
+// acc += a32 (an FP32 accumulator in the FP32 MFMA register layout),
+// remapped through the 64 x 64 LDS scratch X (row pitch `pitch`).  X must be
+// free once every wave arrives; the call starts and ends with a barrier.
+__device__ __forceinline__ void acc32_add(const f32x4 (&a32)[2][2], f64x4 (&acc)[2][2], double* X,
+                                          int pitch) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int wm = (wave >> 1) * 32, wn = (wave & 1) * 32;
+  __syncthreads();
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int j = 0; j < 2; ++j)
+#pragma unroll
+      for (int r = 0; r < 4; ++r)
+        X[(wm + i * 16 + acc_row_f32(lane, r)) * pitch + wn + j * 16 + acc_col(lane)] =
+            (double)a32[i][j][r];
+  __syncthreads();
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int j = 0; j < 2; ++j)
+#pragma unroll
+      for (int r = 0; r < 4; ++r)
+        acc[i][j][r] += X[(wm + i * 16 + acc_row_f64(lane, r)) * pitch + wn + j * 16 + acc_col(lane)];
+  __syncthreads();
+}
+
+__device__ __forceinline__ void cacc32_zero(f32x4 (&acc)[2][2]) {
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int j = 0; j < 2; ++j) acc[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
 }
 
 __device__ __forceinline__ void cacc_zero(f64x4 (&acc)[2][2]) {
@@ -366,10 +405,10 @@ __device__ __forceinline__ void tile_zero(const CTile& t) {
 // TAG gives each kernel its own copy of the non-inlined workers, so that
 // each copy sees a single kind of descriptor pointer (kernel argument or
 // global array) and is optimised for it.
-template <int TAG>
-__device__ void helper_accumulate(const PobtafChain& P, const ContribRange& cr, double sgn,
-                                  bool lower_only, double* XA, double* XB, f64x4 (&acc)[2][2]) {
-  if (cr.k0 >= cr.k1) return;
+template <typename ACC>
+__device__ __forceinline__ void helper_products(const PobtafChain& P, const ContribRange& cr,
+                                                double sgn, bool lower_only, double* XA,
+                                                double* XB, ACC (&acc)[2][2]) {
   TileRegs16 ra, rb;
   auto tileA = [&](int k) { return cr.ra < 0 ? tile_arrow(P, k) : tile_g(P, cr.ra, k); };
   chain_wait(P, factor_flag(P, cr.ra, cr.k0), factor_flag(P, cr.jb, cr.k0));
@@ -392,6 +431,20 @@ __device__ void helper_accumulate(const PobtafChain& P, const ContribRange& cr, 
   __syncthreads();
 }
 
+template <int TAG>
+__device__ void helper_accumulate(const PobtafChain& P, const ContribRange& cr, double sgn,
+                                  bool lower_only, double* XA, double* XB, f64x4 (&acc)[2][2]) {
+  if (cr.k0 >= cr.k1) return;
+  if (P.f32) {
+    f32x4 a32[2][2];
+    cacc32_zero(a32);
+    helper_products(P, cr, sgn, lower_only, XA, XB, a32);
+    acc32_add(a32, acc, XA, KP);
+  } else {
+    helper_products(P, cr, sgn, lower_only, XA, XB, acc);
+  }
+}
+
 // out = acc * Linv(j)^T
 template <int TAG>
 __device__ void helper_solve(const PobtafChain& P, int j, double* XA, double* XB,
@@ -403,7 +456,14 @@ __device__ void helper_solve(const PobtafChain& P, int j, double* XA, double* XB
   tile_store16(rl, XB);      // B[n][k] = Linv(n, k)
   __syncthreads();
   cacc_zero(acc);
-  mfma_abt(XA, XB, acc, 1.0, false);
+  if (P.f32) {
+    f32x4 a32[2][2];
+    cacc32_zero(a32);
+    mfma_abt(XA, XB, a32, 1.0, false);
+    acc32_add(a32, acc, XA, KP);
+  } else {
+    mfma_abt(XA, XB, acc, 1.0, false);
+  }
 }
 
 // ------------------------------------------------------------- chain work
@@ -424,7 +484,24 @@ __device__ void chain_run(const PobtafChain& P, double* S, double* Xs, double* W
     f64x4 acc[2][2];
     if (g > 0) chain_wait(P, P.hflags + t_pd(P, g), nullptr);
     cacc_load(dt, acc);
-    if (g > 0) {
+    if (g > 0 && P.f32) {
+      f32x4 a32[2][2];
+      cacc32_zero(a32);
+      if (wn <= wm) {
+#pragma unroll 4
+        for (int kk = 0; kk < CT; kk += 4) {
+          const double a0 = -S[(wm + li) * TT_LD + kk + lk];
+          const double a1 = -S[(wm + 16 + li) * TT_LD + kk + lk];
+          const double b0 = S[(wn + li) * TT_LD + kk + lk];
+          const double b1 = S[(wn + 16 + li) * TT_LD + kk + lk];
+          a32[0][0] = mfma_acc(a0, b0, a32[0][0]);
+          a32[0][1] = mfma_acc(a0, b1, a32[0][1]);
+          a32[1][0] = mfma_acc(a1, b0, a32[1][0]);
+          a32[1][1] = mfma_acc(a1, b1, a32[1][1]);
+        }
+      }
+      acc32_add(a32, acc, S, TT_LD);
+    } else if (g > 0) {
       // acc -= L(g,g-1) L(g,g-1)^T, with L(g,g-1) in S as S[row][k]
       if (wn <= wm) {
 #pragma unroll 4
@@ -470,16 +547,33 @@ __device__ void chain_run(const PobtafChain& P, double* S, double* Xs, double* W
     }
     __syncthreads();
     cacc_zero(acc);
+    if (P.f32) {
+      f32x4 a32[2][2];
+      cacc32_zero(a32);
 #pragma unroll 4
-    for (int kk = 0; kk < CT; kk += 4) {
-      const double a0 = S[(wm + li) * TT_LD + kk + lk];
-      const double a1 = S[(wm + 16 + li) * TT_LD + kk + lk];
-      const double b0 = Xs[(wn + li) * TT_LD + kk + lk];  // Linv(n, k)
-      const double b1 = Xs[(wn + 16 + li) * TT_LD + kk + lk];
-      acc[0][0] = mfma_f64(a0, b0, acc[0][0]);
-      acc[0][1] = mfma_f64(a0, b1, acc[0][1]);
-      acc[1][0] = mfma_f64(a1, b0, acc[1][0]);
-      acc[1][1] = mfma_f64(a1, b1, acc[1][1]);
+      for (int kk = 0; kk < CT; kk += 4) {
+        const double a0 = S[(wm + li) * TT_LD + kk + lk];
+        const double a1 = S[(wm + 16 + li) * TT_LD + kk + lk];
+        const double b0 = Xs[(wn + li) * TT_LD + kk + lk];  // Linv(n, k)
+        const double b1 = Xs[(wn + 16 + li) * TT_LD + kk + lk];
+        a32[0][0] = mfma_acc(a0, b0, a32[0][0]);
+        a32[0][1] = mfma_acc(a0, b1, a32[0][1]);
+        a32[1][0] = mfma_acc(a1, b0, a32[1][0]);
+        a32[1][1] = mfma_acc(a1, b1, a32[1][1]);
+      }
+      acc32_add(a32, acc, S, TT_LD);
+    } else {
+#pragma unroll 4
+      for (int kk = 0; kk < CT; kk += 4) {
+        const double a0 = S[(wm + li) * TT_LD + kk + lk];
+        const double a1 = S[(wm + 16 + li) * TT_LD + kk + lk];
+        const double b0 = Xs[(wn + li) * TT_LD + kk + lk];  // Linv(n, k)
+        const double b1 = Xs[(wn + 16 + li) * TT_LD + kk + lk];
+        acc[0][0] = mfma_f64(a0, b0, acc[0][0]);
+        acc[0][1] = mfma_f64(a0, b1, acc[0][1]);
+        acc[1][0] = mfma_f64(a1, b0, acc[1][0]);
+        acc[1][1] = mfma_f64(a1, b1, acc[1][1]);
+      }
     }
     cacc_store(xt, acc);
     __syncthreads();  // S no longer read as X
@@ -545,7 +639,7 @@ __device__ __forceinline__ void chain_helper_task(const PobtafChain& P, int task
       __syncthreads();
       mfma_abt(XA, XA, acc, 1.0, true);
     }
-    cacc_store({P.tipp + (int64_t)i * P.a * P.a, P.a, P.a, P.a}, acc);
+    cacc_store({P.tipp + (int64_t)i * P.a * P.a, P.a, P.a, P.a}, acc);  // FP64 (tip path)
     chain_publish(P.hflags + task);
     return;
   }

@@ -18,6 +18,7 @@
 #include "pobtaf_chain.hpp"
 #include "solve_f64.hpp"
 #include "solve_sweep.hpp"
+#include "resid_f64.hpp"
 #include "tile_f64.hpp"
 
 #include <cstdlib>
@@ -116,6 +117,14 @@ struct Recorder {
   }
 };
 thread_local Recorder* g_rec = nullptr;
+// Set for the duration of a mixed-precision entry point: every GEMM problem
+// and the persistent factorization run their products in FP32.
+thread_local bool g_mixed = false;
+struct MixedScope {
+  bool prev;
+  explicit MixedScope(bool on) : prev(g_mixed) { g_mixed = on; }
+  ~MixedScope() { g_mixed = prev; }
+};
 
 int run_gemms(GemmList& list, hipStream_t s) {
   if (g_rec) {
@@ -128,6 +137,7 @@ int run_gemms(GemmList& list, hipStream_t s) {
       p.ksplit = 1;
       p.partial = nullptr;
       p.counters = nullptr;
+      p.f32 = g_mixed ? 1 : 0;
       ph.gemms.push_back(p);
     }
     return ROCBTA_OK;
@@ -147,6 +157,7 @@ int run_gemms(GemmList& list, hipStream_t s) {
     if (p.nterms == 0 && p.beta == 1.0) continue;
     p.tiles_m = cdiv(p.M, GT_M);
     p.tiles_n = cdiv(p.N, GT_N);
+    p.f32 = g_mixed ? 1 : 0;
     if (p.ksplit < 1 || !p.partial || !p.counters) p.ksplit = 1;
     const int64_t tiles = (int64_t)p.tiles_m * p.tiles_n * p.batch1 * p.batch2 * p.ksplit;
     if (g.nprob == ROCBTA_MAX_PROBLEMS || (int64_t)g.total_tiles + tiles > (1ll << 30)) {
@@ -454,6 +465,7 @@ PobtafChain chain_params(int64_t n, int64_t b, int64_t a, double* D, double* L, 
   }
   P.info = info;
   P.spin_limit = 1ll << 22;
+  P.f32 = g_mixed ? 1 : 0;
   return P;
 }
 
@@ -623,6 +635,10 @@ size_t rocbta_workspace_bytes(int op, int64_t n, int64_t b, int64_t a, int64_t r
       return si_layout(n, b, a).total;
     case ROCBTA_OP_POBTAS:
       return sw_layout(n, b, a, std::max<int64_t>(r, 1)).total;
+    case ROCBTA_OP_RESIDUAL: {
+      const int64_t rr = std::max<int64_t>(r, 1), rc = sweep_rc(rr);
+      return align256((size_t)cdiv(rr, rc) * n * cdiv(a, SW_T) * SW_T * rc * sizeof(double)) + 256;
+    }
     default:
       return 0;
   }
@@ -1062,6 +1078,70 @@ int64_t rocbta_dag_selfcheck(int64_t n, int64_t b, int64_t a) {
   if (got != expect) ++bad;
   (void)nwork;
   return bad;
+}
+
+int rocbta_mpobtaf(int64_t n, int64_t b, int64_t a, double* D, double* L, double* R, double* T,
+                   int factorize_last_block, int* info, void* workspace, size_t workspace_bytes,
+                   void* stream) {
+  MixedScope mixed(true);
+  return rocbta_dpobtaf(n, b, a, D, L, R, T, factorize_last_block, info, workspace,
+                        workspace_bytes, stream);
+}
+
+int rocbta_mpobtasi(int64_t n, int64_t b, int64_t a, double* D, double* L, double* R, double* T,
+                    int invert_last_block, void* workspace, size_t workspace_bytes, void* stream) {
+  MixedScope mixed(true);
+  return rocbta_dpobtasi(n, b, a, D, L, R, T, invert_last_block, workspace, workspace_bytes,
+                         stream);
+}
+
+int rocbta_dresidual(int64_t n, int64_t b, int64_t a, const double* D, const double* L,
+                     const double* R, const double* T, const double* X, const double* B,
+                     double* Out, int64_t r, int64_t ld, void* workspace, size_t workspace_bytes,
+                     void* stream) {
+  if (bad_shape(n, b, a) || !D || !X || !B || !Out || (n > 1 && !L) || (a > 0 && (!R || !T)) ||
+      r < 1 || ld < r || Out == X)
+    return ROCBTA_ERR_ARG;
+  if (workspace_bytes < rocbta_workspace_bytes(ROCBTA_OP_RESIDUAL, n, b, a, r) ||
+      (a > 0 && !workspace))
+    return ROCBTA_ERR_WORKSPACE;
+  ResidP P{};
+  P.D = D;
+  P.L = L;
+  P.R = R;
+  P.T = T;
+  P.X = X;
+  P.B = B;
+  P.Out = Out;
+  P.ld = ld;
+  P.n = (int)n;
+  P.b = (int)b;
+  P.a = (int)a;
+  P.nt = cdiv(b, SW_T);
+  P.ta = cdiv(a, SW_T);
+  P.NT = (int)(n * P.nt);
+  P.r = (int)r;
+  P.part = (double*)workspace;
+  const int rc = sweep_rc(r);
+  const int nch = cdiv(r, rc);
+  hipStream_t s = (hipStream_t)stream;
+  const dim3 g1((unsigned)(P.NT + (a > 0 ? n : 0)), (unsigned)nch), g2((unsigned)P.ta, (unsigned)nch);
+  switch (rc) {
+    case 1:
+      residual_rows_kernel<1><<<g1, SW_THREADS, 0, s>>>(P);
+      if (a > 0) residual_tip_kernel<1><<<g2, SW_THREADS, 0, s>>>(P);
+      break;
+    case 4:
+      residual_rows_kernel<4><<<g1, SW_THREADS, 0, s>>>(P);
+      if (a > 0) residual_tip_kernel<4><<<g2, SW_THREADS, 0, s>>>(P);
+      break;
+    default:
+      residual_rows_kernel<8><<<g1, SW_THREADS, 0, s>>>(P);
+      if (a > 0) residual_tip_kernel<8><<<g2, SW_THREADS, 0, s>>>(P);
+      break;
+  }
+  RB_CHECK(hipGetLastError());
+  return ROCBTA_OK;
 }
 
 int rocbta_dgrad(int64_t n, int64_t b, int64_t a, const double* XD, const double* XL,
