@@ -43,6 +43,7 @@ def parse():
     p.add_argument("--cpu-seconds", type=float, default=15.0)
     p.add_argument("--no-solve", action="store_true")
     p.add_argument("--no-batched", action="store_true")
+    p.add_argument("--no-mixed", action="store_true")
     return p.parse_args()
 
 
@@ -207,6 +208,96 @@ def bench_batched(dev, nmat=64, r=4, reps=3, tune=True):
     }
 
 
+def bench_mixed(dev, n, b, a, kappas=(1e4, 1e6, 1e8), r=16, reps=2):
+    """Mixed FP32/FP64 against plain FP64 on the calibrated matrices of the
+    given shape (cond ~ kappa): pobtaf + pobtasi time of both forms, max
+    relative error of the mixed factor, selected inverse and log-determinant
+    against the FP64 results, and the refined solve of r right-hand sides."""
+    import numpy as np
+    import torch
+
+    import serinv_amd.algs as algs
+    from serinv_amd.logdet import pobta_logdet
+    from serinv_amd.mixed import pobtaf_mixed, pobtas_refine, pobtasi_mixed
+    from serinv_amd.utils.conditioning import TABLE, ShiftedBTA
+
+    with open(TABLE) as f:
+        table = json.load(f)
+    sh = ShiftedBTA(n, b, a, dev)
+    stream = torch.cuda.current_stream(dev)
+    rng = np.random.default_rng(64)
+    Bh = torch.from_numpy(rng.uniform(-1.0, 1.0, (n * b + a, r))).to(dev)
+
+    def relerr(x, ref):
+        return float((x - ref).abs().max() / ref.abs().max())
+
+    def timed(fa, fs, blocks):
+        best = (float("inf"), 0.0, 0.0)
+        for _ in range(reps):
+            W = sh.blocks(gamma, [torch.empty_like(u) for u in sh.U]) if blocks is None else blocks
+            torch.cuda.synchronize(dev)
+            e = [torch.cuda.Event(enable_timing=True) for _ in range(3)]
+            e[0].record(stream)
+            fa(*W)
+            e[1].record(stream)
+            F = [x.clone() for x in W]
+            e[2].record(stream)  # the clone is outside both timed ops
+            e3 = torch.cuda.Event(enable_timing=True)
+            e3.record(stream)
+            fs(*W)
+            e4 = torch.cuda.Event(enable_timing=True)
+            e4.record(stream)
+            torch.cuda.synchronize(dev)
+            t = (e[0].elapsed_time(e[1]), e3.elapsed_time(e4))
+            if sum(t) < best[0]:
+                best = (sum(t), t[0], t[1], F, W)
+        return best
+
+    out = []
+    algs.config["check_info"] = False
+    for k in kappas:
+        key = f"{n},{b},{a},{k:.0e}"
+        if key not in table:
+            continue
+        gamma = table[key]["gamma"]
+        A = sh.blocks(gamma)
+        t64, af64, si64, F64, X64 = timed(algs.pobtaf, algs.pobtasi, None)
+        t32, af32, si32, F32, X32 = timed(pobtaf_mixed, pobtasi_mixed, None)
+        ld64 = float(pobta_logdet(F64[0], F64[3]))
+        ld32 = float(pobta_logdet(F32[0], F32[3]))
+        x64 = Bh.clone()
+        algs.pobtas(*F64, x64, trans="N")
+        algs.pobtas(*F64, x64, trans="C")
+        xr = Bh.clone()
+        torch.cuda.synchronize(dev)
+        t0 = time.perf_counter()
+        hist = pobtas_refine(A, F32, xr, max_iter=10, tol=1e-14)
+        torch.cuda.synchronize(dev)
+        t_ref = (time.perf_counter() - t0) * 1e3
+        xm = Bh.clone()
+        algs.pobtas(*F32, xm, trans="N")
+        algs.pobtas(*F32, xm, trans="C")
+        names = "DLRT"
+        out.append({
+            "kappa_target": k, "cond": round(table[key]["cond"], 1), "gamma": gamma,
+            "fp64_ms": round(t64, 3), "mixed_ms": round(t32, 3),
+            "fp64_pobtaf_ms": round(af64, 3), "mixed_pobtaf_ms": round(af32, 3),
+            "fp64_pobtasi_ms": round(si64, 3), "mixed_pobtasi_ms": round(si32, 3),
+            "speedup": round(t64 / t32, 3),
+            "err_L": {nm: float(f"{relerr(x, y):.3e}") for nm, x, y in zip(names, F32, F64) if y.numel()},
+            "err_X": {nm: float(f"{relerr(x, y):.3e}") for nm, x, y in zip(names, X32, X64) if y.numel()},
+            "err_logdet": float(f"{abs(ld32 - ld64) / abs(ld64):.3e}"),
+            "err_x_unrefined": float(f"{relerr(xm, x64):.3e}"),
+            "err_x_refined": float(f"{relerr(xr, x64):.3e}"),
+            "refine_residuals": [float(f"{h:.2e}") for h in hist],
+            "refine_ms": round(t_ref, 2),
+        })
+        del A, F64, X64, F32, X32
+        torch.cuda.empty_cache()
+    algs.config["check_info"] = True
+    return out
+
+
 def main():
     args = parse()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
@@ -349,6 +440,15 @@ def main():
         "roofline": roofline,
         **extra,
     }
+    if not args.no_mixed:
+        print("[bench] mixed FP32/FP64 vs FP64 ...", file=sys.stderr, flush=True)
+        out["mixed"] = bench_mixed(dev, n, b, a)
+        at6 = [m for m in out["mixed"] if m["kappa_target"] == 1e6]
+        out["mixed_speedup"] = at6[0]["speedup"] if at6 else None
+        for m in out["mixed"]:
+            print(f"[bench] mixed kappa {m['kappa_target']:.0e}: {m['mixed_ms']} ms vs {m['fp64_ms']} ms, "
+                  f"err X {max(m['err_X'].values()):.2e}, refined x {m['err_x_refined']:.2e}",
+                  file=sys.stderr, flush=True)
     if not args.no_batched:
         print("[bench] batched plan vs per-matrix loop ...", file=sys.stderr, flush=True)
         out["batched"] = bench_batched(dev)

@@ -132,18 +132,20 @@ def gamma_for(sh, kappa, gamma0, steps=22, iters=60):
     return best
 
 
-def calibrate(n, b, a, kappas, device, seed=63):
+def calibrate(n, b, a, kappas, device, seed=63, path=TABLE):
     sh = ShiftedBTA(n, b, a, device, seed)
     g0 = pd_boundary(sh)
+    print(f"[calibrate] n={n} b={b} a={a}: PD boundary gamma0 = {g0:.12g}", flush=True)
     try:
-        with open(TABLE) as f:
+        with open(path) as f:
             table = json.load(f)
     except (OSError, ValueError):
         table = {}
     for k in kappas:
         gamma, c = gamma_for(sh, k, g0)
+        print(f"[calibrate] target {k:.0e}: gamma = {gamma:.12g}, cond = {c:.3e}", flush=True)
         table[f"{n},{b},{a},{k:.0e}"] = {"gamma": gamma, "cond": c, "gamma_pd": g0, "seed": seed}
-    with open(TABLE, "w") as f:
+    with open(path, "w") as f:
         json.dump(table, f, indent=1, sort_keys=True)
     return table
 
@@ -156,6 +158,7 @@ if __name__ == "__main__":
     p.add_argument("--b", type=int, default=1024)
     p.add_argument("--a", type=int, default=32)
     p.add_argument("--kappa", type=float, nargs="+", default=[1e4, 1e6, 1e8])
+    p.add_argument("--out", default=TABLE)
     args = p.parse_args()
-    out = calibrate(args.n, args.b, args.a, args.kappa, torch.device("cuda:0"))
+    out = calibrate(args.n, args.b, args.a, args.kappa, torch.device("cuda:0"), path=args.out)
     print(json.dumps(out, indent=1))
