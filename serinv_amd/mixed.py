@@ -73,7 +73,8 @@ def pobtas_refine(A, F, B, max_iter=10, tol=1e-13):
 
     Returns the list of relative residual norms ||B - A x||_F / ||B||_F after
     each correction (the first entry is the plain solve).  Stops when it drops
-    below tol, after max_iter corrections, or when it stops decreasing.
+    below tol, after max_iter corrections, or when a correction no longer
+    halves it (the FP64 rounding floor of the residual has been reached).
     """
     D, L, R, T = F
     B2 = _tensor(B, "B").reshape(B.shape[0], -1)
@@ -87,7 +88,7 @@ def pobtas_refine(A, F, B, max_iter=10, tol=1e-13):
     for _ in range(max_iter + 1):
         residual(A, x, b0, out=res)
         rel = float(res.norm()) / bnorm
-        if hist and rel >= hist[-1]:
+        if hist and rel > 0.5 * hist[-1]:
             hist.append(rel)
             break
         hist.append(rel)
