@@ -250,6 +250,15 @@ __device__ __forceinline__ void chain_publish(int* f) {
   __syncthreads();  // all waves' stores of the tile have completed
   if (threadIdx.x == 0) __hip_atomic_store(f, 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
 }
+// Two flags behind one release.
+__device__ __forceinline__ void chain_publish2(int* f1, int* f2) {
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+    __hip_atomic_store(f1, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_store(f2, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  }
+}
 
 // ----------------------------------------------------------- tile movement
 typedef double cdbl2 __attribute__((ext_vector_type(2)));
@@ -529,8 +538,13 @@ __device__ void chain_run(const PobtafChain& P, double* S, double* Xs, double* W
     lds_trtri(S2, X2, W2, m);
     double* li_out = P.linv + (int64_t)g * CT * CT;
     for (int idx = tid; idx < CT * CT; idx += CH_THREADS) li_out[idx] = X2[idx >> 6][idx & 63];
-    chain_publish(P.cflags + 2 * g);
-    if (g + 1 >= band_end(P, g)) continue;
+    if (g + 1 >= band_end(P, g)) {
+      chain_publish(P.cflags + 2 * g);
+      continue;
+    }
+    // L(g,g) and Linv(g) are published together with L(g+1,g) at the end of
+    // the step: one release instead of two on the chain's path (no helper
+    // task feeding PX(g) needs Linv(g)).
     // L(g+1,g) = PX(g) * Linv(g)^T
     chain_wait(P, P.hflags + t_px(P, g), nullptr);
     const CTile xt = tile_g(P, g + 1, g);
@@ -578,7 +592,7 @@ __device__ void chain_run(const PobtafChain& P, double* S, double* Xs, double* W
     cacc_store(xt, acc);
     __syncthreads();  // S no longer read as X
     cacc_to_lds(acc, S, TT_LD);  // keep L(g+1,g) as S[row][k]
-    chain_publish(P.cflags + 2 * g + 1);
+    chain_publish2(P.cflags + 2 * g, P.cflags + 2 * g + 1);
   }
 }
 
