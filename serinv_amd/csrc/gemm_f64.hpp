@@ -71,18 +71,32 @@ struct GemmGroup {
 };
 
 // ------------------------------------------------------------------ staging
-// A pipeline stage holds a 64 x GT_K slice of alpha*op(A) and of op(B)^T
+// Two tile shapes: TS = 64 (64 x 64 output tile, K stage 32; four 16x16
+// MFMA accumulators per wave) for small and many-problem launches, and
+// TS = 128 (128 x 128 tile, K stage 16; sixteen accumulators per wave, one
+// workgroup per CU) for large products, where the wider tile halves the
+// operand traffic per flop and gives each wave 16 independent MFMA chains.
+//
+// A pipeline stage holds a TS x K slice of alpha*op(A) and of op(B)^T
 // (rows m resp. n, columns k).  Each operand is read from global memory in
 // its natural orientation with 16-byte loads (two doubles per lane, whole
-// 256/512-byte row segments per wave) and kept in LDS in that orientation:
+// row segments per wave) and kept in LDS in that orientation:
 //   K-contiguous source (A not transposed, B transposed): X[r][k], pitch PK
 //   R-contiguous source (A transposed,     B not transposed): X[k][r], pitch PR
 // An MFMA operand fetch (16 consecutive r at one k, four k per wave) then
 // reads LDS with a small stride in either layout.
-constexpr int PK = GT_K + 2;    // 34 doubles: rows of a K-contiguous slice
-constexpr int PR = GT_M + 2;    // 66 doubles: rows of an R-contiguous slice
-constexpr int SLICE = 64 * PK;  // doubles per operand slot (>= GT_K * PR)
-static_assert(GT_K * PR <= SLICE, "LDS slot too small");
+template <int TS>
+struct GTile {
+  static constexpr int K = (TS == 64) ? 32 : 16;
+  static constexpr int PK = K + 2;
+  static constexpr int PR = TS + 2;
+  static constexpr int SLICE = (TS * PK > K * PR) ? TS * PK : K * PR;
+  static constexpr int NB = TS / 32;  // 16x16 blocks per side of a wave's quadrant
+};
+constexpr int PK = GTile<64>::PK;
+constexpr int PR = GTile<64>::PR;
+constexpr int SLICE = GTile<64>::SLICE;
+static_assert(GTile<64>::K == GT_K, "64-tile stage depth");
 
 typedef double dbl2 __attribute__((ext_vector_type(2)));
 
@@ -91,17 +105,18 @@ struct StageRegs {
 };
 
 // Global -> registers.  ld: row stride of the stored matrix.  For a
-// K-contiguous slice the stored rows are r (64 of them), columns k; for an
-// R-contiguous slice the stored rows are k (GT_K of them), columns r.
+// K-contiguous slice the stored rows are r (TS of them), columns k; for an
+// R-contiguous slice the stored rows are k (K of them), columns r.
 // rows_ok / cols_ok bound the stored row / column index (exclusive).
-template <bool KCONT, bool FAST>
+template <int TS, bool KCONT, bool FAST>
 __device__ __forceinline__ void slice_load(const double* base, int64_t ld, int row0, int col0,
                                            int rows_ok, int cols_ok, double alpha,
                                            StageRegs& r) {
+  constexpr int TPR = KCONT ? GTile<TS>::K / 2 : TS / 2;  // threads per stored row
   const int tid = threadIdx.x;
-  const int rsub = KCONT ? (tid >> 4) : (tid >> 5);
-  const int c = KCONT ? (tid & 15) * 2 : (tid & 31) * 2;
-  const int rstep = KCONT ? 16 : 8;
+  const int rsub = tid / TPR;
+  const int c = (tid % TPR) * 2;
+  constexpr int rstep = GT_THREADS / TPR;
 #pragma unroll
   for (int q = 0; q < 4; ++q) {
     const int row = row0 + rsub + rstep * q, col = col0 + c;
@@ -117,34 +132,36 @@ __device__ __forceinline__ void slice_load(const double* base, int64_t ld, int r
   }
 }
 
-template <bool KCONT>
+template <int TS, bool KCONT>
 __device__ __forceinline__ void slice_store(const StageRegs& r, double* X) {
+  constexpr int TPR = KCONT ? GTile<TS>::K / 2 : TS / 2;
   const int tid = threadIdx.x;
-  const int rsub = KCONT ? (tid >> 4) : (tid >> 5);
-  const int c = KCONT ? (tid & 15) * 2 : (tid & 31) * 2;
-  const int rstep = KCONT ? 16 : 8;
-  const int pitch = KCONT ? PK : PR;
+  const int rsub = tid / TPR;
+  const int c = (tid % TPR) * 2;
+  constexpr int rstep = GT_THREADS / TPR;
+  constexpr int pitch = KCONT ? GTile<TS>::PK : GTile<TS>::PR;
 #pragma unroll
   for (int q = 0; q < 4; ++q)
     *reinterpret_cast<dbl2*>(X + (rsub + rstep * q) * pitch + c) = r.v[q];
 }
 
 // Operand element (r, k) of a staged slice.
-template <bool KCONT>
+template <int TS, bool KCONT>
 __device__ __forceinline__ double slice_at(const double* X, int r, int k) {
-  return KCONT ? X[r * PK + k] : X[k * PR + r];
+  return KCONT ? X[r * GTile<TS>::PK + k] : X[k * GTile<TS>::PR + r];
 }
 
 // K range of a term restricted by the triangular shape of its operands.
+template <int TS>
 __device__ __forceinline__ void gemm_k_range(const GemmTerm& t, int m0, int n0, int& kbeg,
                                              int& kend) {
   kbeg = 0;
   kend = t.K;
-  if (t.triA == TRI_LOWER) kend = min(kend, m0 + GT_M);
+  if (t.triA == TRI_LOWER) kend = min(kend, m0 + TS);
   if (t.triA == TRI_UPPER) kbeg = max(kbeg, m0);
   if (t.triB == TRI_LOWER) kbeg = max(kbeg, n0);
-  if (t.triB == TRI_UPPER) kend = min(kend, n0 + GT_N);
-  kbeg = (kbeg / GT_K) * GT_K;
+  if (t.triB == TRI_UPPER) kend = min(kend, n0 + TS);
+  kbeg = (kbeg / GTile<TS>::K) * GTile<TS>::K;
 }
 
 __device__ __forceinline__ bool aligned16(const double* p, int64_t ld) {
@@ -152,8 +169,8 @@ __device__ __forceinline__ bool aligned16(const double* p, int64_t ld) {
 }
 
 // One term's operands for one workgroup tile.  FAST: every stage is a full,
-// 16-byte aligned 64 x GT_K slice (no bounds checks, vector loads only).
-template <bool TA, bool TB, bool FAST>
+// 16-byte aligned TS x K slice (no bounds checks, vector loads only).
+template <int TS, bool TA, bool TB, bool FAST>
 struct TermView {
   const double* A;
   const double* B;
@@ -164,104 +181,111 @@ struct TermView {
   __device__ __forceinline__ void load(int k0, StageRegs& ra, StageRegs& rb) const {
     // A: K-contiguous when not transposed (stored M x K), else R-contiguous (stored K x M)
     if (!TA)
-      slice_load<true, FAST>(A, lda, m0, k0, M, ke, alpha, ra);
+      slice_load<TS, true, FAST>(A, lda, m0, k0, M, ke, alpha, ra);
     else
-      slice_load<false, FAST>(A, lda, k0, m0, ke, M, alpha, ra);
+      slice_load<TS, false, FAST>(A, lda, k0, m0, ke, M, alpha, ra);
     // B: op(B)(k, n); transposed => stored N x K (K-contiguous), else K x N
     if (TB)
-      slice_load<true, FAST>(B, ldb, n0, k0, N, ke, 1.0, rb);
+      slice_load<TS, true, FAST>(B, ldb, n0, k0, N, ke, 1.0, rb);
     else
-      slice_load<false, FAST>(B, ldb, k0, n0, ke, N, 1.0, rb);
+      slice_load<TS, false, FAST>(B, ldb, k0, n0, ke, N, 1.0, rb);
   }
   __device__ __forceinline__ void store(const StageRegs& ra, const StageRegs& rb, double* XA,
                                         double* XB) const {
-    slice_store<!TA>(ra, XA);
-    slice_store<TB>(rb, XB);
+    slice_store<TS, !TA>(ra, XA);
+    slice_store<TS, TB>(rb, XB);
   }
 };
 
-// One LDS stage of MFMAs (wave quadrant 32x32 = 2x2 MFMA tiles).
-template <bool TA, bool TB, typename ACC>
+// One LDS stage of MFMAs (wave quadrant (TS/2)^2 = NB x NB MFMA tiles).
+template <int TS, bool TA, bool TB, typename ACC>
 __device__ __forceinline__ void gemm_mfma_stage(const double* XA, const double* XB,
-                                                ACC (&acc)[2][2]) {
+                                                ACC (&acc)[GTile<TS>::NB][GTile<TS>::NB]) {
+  constexpr int NB = GTile<TS>::NB;
   const int lane = threadIdx.x & 63;
   const int wave = threadIdx.x >> 6;
-  const int wm = (wave >> 1) * 32, wn = (wave & 1) * 32;
+  const int wm = (wave >> 1) * (TS / 2), wn = (wave & 1) * (TS / 2);
   const int li = lane & 15, lk = lane >> 4;
 #pragma unroll
-  for (int kk = 0; kk < GT_K; kk += 4) {
-    const double a0 = slice_at<!TA>(XA, wm + li, kk + lk);
-    const double a1 = slice_at<!TA>(XA, wm + 16 + li, kk + lk);
-    const double b0 = slice_at<TB>(XB, wn + li, kk + lk);
-    const double b1 = slice_at<TB>(XB, wn + 16 + li, kk + lk);
-    acc[0][0] = mfma_acc(a0, b0, acc[0][0]);
-    acc[0][1] = mfma_acc(a0, b1, acc[0][1]);
-    acc[1][0] = mfma_acc(a1, b0, acc[1][0]);
-    acc[1][1] = mfma_acc(a1, b1, acc[1][1]);
+  for (int kk = 0; kk < GTile<TS>::K; kk += 4) {
+    double av[NB], bv[NB];
+#pragma unroll
+    for (int i = 0; i < NB; ++i) av[i] = slice_at<TS, !TA>(XA, wm + 16 * i + li, kk + lk);
+#pragma unroll
+    for (int j = 0; j < NB; ++j) bv[j] = slice_at<TS, TB>(XB, wn + 16 * j + li, kk + lk);
+#pragma unroll
+    for (int i = 0; i < NB; ++i)
+#pragma unroll
+      for (int j = 0; j < NB; ++j) acc[i][j] = mfma_acc(av[i], bv[j], acc[i][j]);
   }
 }
 
 // acc += alpha * op(A) op(B) over the K range [kb, ke): LDS double buffer
 // (slots 0/1) and two register stages in flight (x/y), unrolled by two so
 // that every register set has a fixed role.
-template <bool TA, bool TB, bool FAST, typename ACC>
-__device__ __forceinline__ void gemm_term_pipeline(const TermView<TA, TB, FAST>& v, int kb,
-                                                   double* lds, ACC (&acc)[2][2]) {
+template <int TS, bool TA, bool TB, bool FAST, typename ACC>
+__device__ __forceinline__ void gemm_term_pipeline(const TermView<TS, TA, TB, FAST>& v, int kb,
+                                                   double* lds,
+                                                   ACC (&acc)[GTile<TS>::NB][GTile<TS>::NB]) {
+  constexpr int KS = GTile<TS>::K, SL = GTile<TS>::SLICE;
   const int ke = v.ke;
-  const int nst = kb < ke ? (ke - kb + GT_K - 1) / GT_K : 0;
+  const int nst = kb < ke ? (ke - kb + KS - 1) / KS : 0;
   if (nst == 0) return;
   double* A0 = lds;
-  double* B0 = lds + SLICE;
-  double* A1 = lds + 2 * SLICE;
-  double* B1 = lds + 3 * SLICE;
+  double* B0 = lds + SL;
+  double* A1 = lds + 2 * SL;
+  double* B1 = lds + 3 * SL;
   StageRegs xa, xb, ya, yb;
   __syncthreads();  // LDS free (previous term / previous use)
   v.load(kb, xa, xb);
-  if (nst > 1) v.load(kb + GT_K, ya, yb);
+  if (nst > 1) v.load(kb + KS, ya, yb);
   v.store(xa, xb, A0, B0);
   __syncthreads();
-  if (nst > 2) v.load(kb + 2 * GT_K, xa, xb);
+  if (nst > 2) v.load(kb + 2 * KS, xa, xb);
   for (int s = 0; s < nst; s += 2) {
     // slot 0 holds stage s; y holds stage s+1; x holds stage s+2 (in flight)
-    gemm_mfma_stage<TA, TB, ACC>(A0, B0, acc);
+    gemm_mfma_stage<TS, TA, TB, ACC>(A0, B0, acc);
     if (s + 1 >= nst) break;
     v.store(ya, yb, A1, B1);
     __syncthreads();
-    if (s + 3 < nst) v.load(kb + (s + 3) * GT_K, ya, yb);
-    gemm_mfma_stage<TA, TB, ACC>(A1, B1, acc);
+    if (s + 3 < nst) v.load(kb + (s + 3) * KS, ya, yb);
+    gemm_mfma_stage<TS, TA, TB, ACC>(A1, B1, acc);
     if (s + 2 >= nst) break;
     v.store(xa, xb, A0, B0);
     __syncthreads();
-    if (s + 4 < nst) v.load(kb + (s + 4) * GT_K, xa, xb);
+    if (s + 4 < nst) v.load(kb + (s + 4) * KS, xa, xb);
   }
 }
 
-// Stages [sb, se) of the term (a stage = GT_K consecutive k from the term's
+// Stages [sb, se) of the term (a stage = K consecutive k from the term's
 // first k); the K range of those stages is clipped to the term's range.
-template <bool TA, bool TB, typename ACC>
+template <int TS, bool TA, bool TB, typename ACC>
 __device__ __forceinline__ void gemm_term(const GemmTerm& t, const double* A, const double* B,
                                           int M, int N, int m0, int n0, int sb, int se,
-                                          double* lds, ACC (&acc)[2][2]) {
+                                          double* lds, ACC (&acc)[GTile<TS>::NB][GTile<TS>::NB]) {
+  constexpr int KS = GTile<TS>::K;
   int kb, ke;
-  gemm_k_range(t, m0, n0, kb, ke);
-  ke = min(ke, kb + se * GT_K);
-  kb = kb + sb * GT_K;
+  gemm_k_range<TS>(t, m0, n0, kb, ke);
+  ke = min(ke, kb + se * KS);
+  kb = kb + sb * KS;
   if (kb >= ke) return;
-  const bool fast = aligned16(A, t.lda) && aligned16(B, t.ldb) && m0 + GT_M <= M &&
-                    n0 + GT_N <= N && ((ke - kb) % GT_K) == 0;
+  const bool fast = aligned16(A, t.lda) && aligned16(B, t.ldb) && m0 + TS <= M &&
+                    n0 + TS <= N && ((ke - kb) % KS) == 0;
   if (fast) {
-    TermView<TA, TB, true> v{A, B, t.lda, t.ldb, M, N, m0, n0, ke, t.alpha};
-    gemm_term_pipeline<TA, TB, true, ACC>(v, kb, lds, acc);
+    TermView<TS, TA, TB, true> v{A, B, t.lda, t.ldb, M, N, m0, n0, ke, t.alpha};
+    gemm_term_pipeline<TS, TA, TB, true, ACC>(v, kb, lds, acc);
   } else {
-    TermView<TA, TB, false> v{A, B, t.lda, t.ldb, M, N, m0, n0, ke, t.alpha};
-    gemm_term_pipeline<TA, TB, false, ACC>(v, kb, lds, acc);
+    TermView<TS, TA, TB, false> v{A, B, t.lda, t.ldb, M, N, m0, n0, ke, t.alpha};
+    gemm_term_pipeline<TS, TA, TB, false, ACC>(v, kb, lds, acc);
   }
 }
 
-// One 64x64 output tile (global tile index `tile`) of problem P, with
-// accumulators of type ACC (f64x4: FP64, f32x4: mixed precision).
-template <typename ACC>
+// One TS x TS output tile (global tile index `tile`) of problem P, with
+// accumulators of type ACC (f64x4: FP64, f32x4: mixed precision).  The
+// problem's tiles_m / tiles_n / tile_begin were computed for this TS.
+template <int TS, typename ACC>
 __device__ __forceinline__ void gemm_tile_acc(const GemmProblem& P, int tile, double* lds) {
+  constexpr int NB = GTile<TS>::NB, KS = GTile<TS>::K;
   int local = (tile - P.tile_begin);
   const int split = local % P.ksplit;
   local /= P.ksplit;
@@ -271,27 +295,27 @@ __device__ __forceinline__ void gemm_tile_acc(const GemmProblem& P, int tile, do
   local -= bidx * per_batch;
   const int b1 = bidx % P.batch1, b2 = bidx / P.batch1;
   const int tm = local / P.tiles_n, tn = local % P.tiles_n;
-  const int m0 = tm * GT_M, n0 = tn * GT_N;
-  if (P.uplo == TRI_LOWER && n0 > m0 + GT_M - 1) return;
-  if (P.uplo == TRI_UPPER && m0 > n0 + GT_N - 1) return;
+  const int m0 = tm * TS, n0 = tn * TS;
+  if (P.uplo == TRI_LOWER && n0 > m0 + TS - 1) return;
+  if (P.uplo == TRI_UPPER && m0 > n0 + TS - 1) return;
 
   double* C = P.C + b1 * P.sC1 + b2 * P.sC2;
   const int lane = threadIdx.x & 63;
   const int wave = threadIdx.x >> 6;
-  const int wm = (wave >> 1) * 32, wn = (wave & 1) * 32;
+  const int wm = (wave >> 1) * (TS / 2), wn = (wave & 1) * (TS / 2);
 
-  ACC acc[2][2];
+  ACC acc[NB][NB];
 #pragma unroll
-  for (int i = 0; i < 2; ++i)
+  for (int i = 0; i < NB; ++i)
 #pragma unroll
-    for (int j = 0; j < 2; ++j) acc[i][j] = ACC{0, 0, 0, 0};
+    for (int j = 0; j < NB; ++j) acc[i][j] = ACC{0, 0, 0, 0};
 
   // stage counts of the terms; this workgroup's share of the stage sequence
   int nst[2] = {0, 0};
   for (int ti = 0; ti < P.nterms; ++ti) {
     int kb, ke;
-    gemm_k_range(P.t[ti], m0, n0, kb, ke);
-    nst[ti] = kb < ke ? (ke - kb + GT_K - 1) / GT_K : 0;
+    gemm_k_range<TS>(P.t[ti], m0, n0, kb, ke);
+    nst[ti] = kb < ke ? (ke - kb + KS - 1) / KS : 0;
   }
   const int total = nst[0] + nst[1];
   const int g0 = (int)((int64_t)total * split / P.ksplit);
@@ -304,26 +328,26 @@ __device__ __forceinline__ void gemm_tile_acc(const GemmProblem& P, int tile, do
     const double* A = t.A + b1 * t.sA1 + b2 * t.sA2;
     const double* B = t.B + b1 * t.sB1 + b2 * t.sB2;
     if (!t.transA && !t.transB)
-      gemm_term<false, false, ACC>(t, A, B, P.M, P.N, m0, n0, sb, se, lds, acc);
+      gemm_term<TS, false, false, ACC>(t, A, B, P.M, P.N, m0, n0, sb, se, lds, acc);
     else if (!t.transA && t.transB)
-      gemm_term<false, true, ACC>(t, A, B, P.M, P.N, m0, n0, sb, se, lds, acc);
+      gemm_term<TS, false, true, ACC>(t, A, B, P.M, P.N, m0, n0, sb, se, lds, acc);
     else if (t.transA && !t.transB)
-      gemm_term<true, false, ACC>(t, A, B, P.M, P.N, m0, n0, sb, se, lds, acc);
+      gemm_term<TS, true, false, ACC>(t, A, B, P.M, P.N, m0, n0, sb, se, lds, acc);
     else
-      gemm_term<true, true, ACC>(t, A, B, P.M, P.N, m0, n0, sb, se, lds, acc);
+      gemm_term<TS, true, true, ACC>(t, A, B, P.M, P.N, m0, n0, sb, se, lds, acc);
   }
 
   if (P.ksplit > 1) {
     // Publish this split's partial tile; the last finisher sums all of them
     // in split order (deterministic) and runs the epilogue.
     const int tid = threadIdx.x;
-    double* mine = P.partial + ((int64_t)tile_id * P.ksplit + split) * (GT_M * GT_N);
+    double* mine = P.partial + ((int64_t)tile_id * P.ksplit + split) * (TS * TS);
 #pragma unroll
-    for (int i = 0; i < 2; ++i)
+    for (int i = 0; i < NB; ++i)
 #pragma unroll
-      for (int j = 0; j < 2; ++j)
+      for (int j = 0; j < NB; ++j)
 #pragma unroll
-        for (int r = 0; r < 4; ++r) mine[((i * 2 + j) * 4 + r) * GT_THREADS + tid] = acc[i][j][r];
+        for (int r = 0; r < 4; ++r) mine[((i * NB + j) * 4 + r) * GT_THREADS + tid] = acc[i][j][r];
     __syncthreads();
     __shared__ int s_last;
     if (tid == 0) {
@@ -335,25 +359,25 @@ __device__ __forceinline__ void gemm_tile_acc(const GemmProblem& P, int tile, do
     }
     __syncthreads();
     if (!s_last) return;
-    const double* base = P.partial + (int64_t)tile_id * P.ksplit * (GT_M * GT_N);
+    const double* base = P.partial + (int64_t)tile_id * P.ksplit * (TS * TS);
 #pragma unroll
-    for (int i = 0; i < 2; ++i)
+    for (int i = 0; i < NB; ++i)
 #pragma unroll
-      for (int j = 0; j < 2; ++j)
+      for (int j = 0; j < NB; ++j)
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
-          const int idx = ((i * 2 + j) * 4 + r) * GT_THREADS + tid;
+          const int idx = ((i * NB + j) * 4 + r) * GT_THREADS + tid;
           double sum = 0.0;
           for (int sp = 0; sp < P.ksplit; ++sp)
-            sum += (sp == split) ? (double)acc[i][j][r] : base[(int64_t)sp * (GT_M * GT_N) + idx];
+            sum += (sp == split) ? (double)acc[i][j][r] : base[(int64_t)sp * (TS * TS) + idx];
           acc[i][j][r] = sum;
         }
   }
 
 #pragma unroll
-  for (int i = 0; i < 2; ++i)
+  for (int i = 0; i < NB; ++i)
 #pragma unroll
-    for (int j = 0; j < 2; ++j)
+    for (int j = 0; j < NB; ++j)
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
         const int row = m0 + wm + i * 16 + acc_row(acc[i][j], lane, r);
@@ -369,33 +393,44 @@ __device__ __forceinline__ void gemm_tile_acc(const GemmProblem& P, int tile, do
       }
 }
 
+template <int TS>
 __device__ __forceinline__ void gemm_tile(const GemmProblem& P, int tile, double* lds) {
   if (P.f32)
-    gemm_tile_acc<f32x4>(P, tile, lds);
+    gemm_tile_acc<TS, f32x4>(P, tile, lds);
   else
-    gemm_tile_acc<f64x4>(P, tile, lds);
+    gemm_tile_acc<TS, f64x4>(P, tile, lds);
 }
 
 __global__ void __launch_bounds__(GT_THREADS, 2) gemm_group_kernel(GemmGroup g) {
-  __shared__ __attribute__((aligned(16))) double lds[4 * SLICE];
+  __shared__ __attribute__((aligned(16))) double lds[4 * GTile<64>::SLICE];
   const int tile = blockIdx.x;
   int pi = 0;
   while (pi + 1 < g.nprob && tile >= g.p[pi + 1].tile_begin) ++pi;
-  gemm_tile(g.p[pi], tile, lds);
+  gemm_tile<64>(g.p[pi], tile, lds);
 }
 
-// The same over a device array of problems (sorted by tile_begin): used by
-// the batched plans, whose launch lists are built once and replayed.
+// 128 x 128 tiles for large products: one workgroup per CU, so that each
+// wave can hold its sixteen accumulators without spilling.
+__global__ void __launch_bounds__(GT_THREADS, 1) gemm_group_kernel_t128(GemmGroup g) {
+  __shared__ __attribute__((aligned(16))) double lds[4 * GTile<128>::SLICE];
+  const int tile = blockIdx.x;
+  int pi = 0;
+  while (pi + 1 < g.nprob && tile >= g.p[pi + 1].tile_begin) ++pi;
+  gemm_tile<128>(g.p[pi], tile, lds);
+}
+
+// The same over a device array of problems (sorted by tile_begin, 64-tiles):
+// used by the batched plans, whose launch lists are built once and replayed.
 __global__ void __launch_bounds__(GT_THREADS, 2) gemm_list_kernel(const GemmProblem* probs,
                                                                   int nprob) {
-  __shared__ __attribute__((aligned(16))) double lds[4 * SLICE];
+  __shared__ __attribute__((aligned(16))) double lds[4 * GTile<64>::SLICE];
   const int tile = blockIdx.x;
   int lo = 0, hi = nprob - 1;
   while (lo < hi) {
     const int mid = (lo + hi + 1) >> 1;
     if (probs[mid].tile_begin <= tile) lo = mid; else hi = mid - 1;
   }
-  gemm_tile(probs[lo], tile, lds);
+  gemm_tile<64>(probs[lo], tile, lds);
 }
 
 }  // namespace rocbta

@@ -126,6 +126,28 @@ struct MixedScope {
   ~MixedScope() { g_mixed = prev; }
 };
 
+// Output tile size for a launch list.  64 by default: measured on the
+// selected inversion at n=256, b=1024 the 128-tile kernel (one workgroup
+// per CU, split-K 4) took 107.6 ms against 62.1 ms for the 64-tile kernel
+// (two workgroups per CU).  ROCBTA_GEMM_TILE=128 selects it for lists whose
+// largest problem is large in both output dimensions and in K.
+bool gemm_tile128_enabled() {
+  const char* e = getenv("ROCBTA_GEMM_TILE");
+  return e && strcmp(e, "128") == 0;
+}
+int gemm_ts_for(int M, int N, int K) {
+  return (gemm_tile128_enabled() && M >= 256 && N >= 256 && K >= 256) ? 128 : 64;
+}
+int gemm_ts(const GemmList& list) {
+  int ts = 64;
+  for (const GemmProblem& p : list.probs) {
+    int kmax = 0;
+    for (int t = 0; t < p.nterms; ++t) kmax = std::max(kmax, p.t[t].K);
+    if (gemm_ts_for(p.M, p.N, kmax) == 128) ts = 128;
+  }
+  return ts;
+}
+
 int run_gemms(GemmList& list, hipStream_t s) {
   if (g_rec) {
     Phase& ph = g_rec->next();
@@ -142,11 +164,15 @@ int run_gemms(GemmList& list, hipStream_t s) {
     }
     return ROCBTA_OK;
   }
+  const int ts = gemm_ts(list);
   GemmGroup g{};
   auto flush = [&]() -> int {
     if (g.nprob == 0) return ROCBTA_OK;
     if (g.total_tiles > 0) {
-      gemm_group_kernel<<<g.total_tiles, GT_THREADS, 0, s>>>(g);
+      if (ts == 128)
+        gemm_group_kernel_t128<<<g.total_tiles, GT_THREADS, 0, s>>>(g);
+      else
+        gemm_group_kernel<<<g.total_tiles, GT_THREADS, 0, s>>>(g);
       RB_CHECK(hipGetLastError());
     }
     g = GemmGroup{};
@@ -155,8 +181,8 @@ int run_gemms(GemmList& list, hipStream_t s) {
   for (GemmProblem p : list.probs) {
     if (p.M <= 0 || p.N <= 0 || p.batch1 <= 0 || p.batch2 <= 0) continue;
     if (p.nterms == 0 && p.beta == 1.0) continue;
-    p.tiles_m = cdiv(p.M, GT_M);
-    p.tiles_n = cdiv(p.N, GT_N);
+    p.tiles_m = cdiv(p.M, ts);
+    p.tiles_n = cdiv(p.N, ts);
     p.f32 = g_mixed ? 1 : 0;
     if (p.ksplit < 1 || !p.partial || !p.counters) p.ksplit = 1;
     const int64_t tiles = (int64_t)p.tiles_m * p.tiles_n * p.batch1 * p.batch2 * p.ksplit;
@@ -381,7 +407,10 @@ SiLayout si_layout(int64_t n, int64_t b, int64_t a) {
   l.w = take((size_t)nch * ntb * TT * TT);
   l.tinv = take((size_t)a * a);
   l.wt = take((size_t)nta * TT * TT);
-  l.part = take((size_t)(ntb * ntb + ntb) * 2 * TT * TT);  // split-K partial tiles (2 splits)
+  // split-K partial tiles: up to 2 splits of 64-tiles or 8 splits of 128-tiles
+  const int64_t t128 = (b + 127) / 128, a128 = (a + 127) / 128;
+  l.part = take((size_t)std::max<int64_t>((ntb * ntb + ntb * nta + ntb) * 2 * TT * TT,
+                                          (t128 * t128 + t128 * a128 + t128) * 8 * 128 * 128));
   l.cnt = take((size_t)(ntb * ntb + ntb + 64) / 2 + 1);  // split-K tile counters (ints)
   l.total = off;
   return l;
@@ -718,16 +747,24 @@ int rocbta_dpobtasi(int64_t n, int64_t b, int64_t a, double* D, double* L, doubl
   double* part = (double*)(ws + lay.part);
   int* cnt = (int*)(ws + lay.cnt);
   const int ntb = cdiv(b, TT);
-  // Split the K loop of the recurrence's b x b products in two when one
-  // product alone has too few tiles to give every CU two workgroups.
-  const int ksplit = (ntb * ntb < 2 * device_cus() && b >= 256) ? 2 : 1;
+  // Split the K loop of the recurrence's b x b products when one product
+  // alone has too few output tiles to occupy every CU (128-tiles: one
+  // workgroup per CU; 64-tiles: two).
+  const int ts = gemm_ts_for((int)b, (int)b, (int)b);
+  const int nts = cdiv(b, ts);
+  const int cus = device_cus();
+  int ksplit = 1;
+  if (ts == 128)
+    ksplit = std::max(1, std::min(8, cus / (nts * nts)));
+  else if (nts * nts < 2 * cus && b >= 256)
+    ksplit = 2;
   if (ksplit > 1) {
     fill_i32_kernel<<<1, 256, 0, s>>>(cnt, ntb * ntb + ntb, 0);
     RB_CHECK(hipGetLastError());
   }
   // the arrow problem of a step has its own split-K bookkeeping
-  double* part_r = part + (int64_t)ntb * ntb * 2 * TT * TT;
-  int* cnt_r = cnt + ntb * ntb;
+  double* part_r = part + (int64_t)nts * nts * ksplit * ts * ts;
+  int* cnt_r = cnt + nts * nts;
   const int bi = (int)b, ai = (int)a;
   const int64_t bb = b * b, ab = a * b;
   // Blocks whose factor is still stored: all of them, or all but the last

@@ -30,12 +30,17 @@ def algs(gpu_device):
 def pobtaf_algo(request, monkeypatch):
     """Run the test with the persistent kernels (chain pobtaf, sweep pobtas)
     and with the launch-per-tile forms (ROCBTA_POBTAF_ALGO=loop,
-    ROCBTA_POBTAS_ALGO=loop)."""
+    ROCBTA_POBTAS_ALGO=loop); the launch-per-tile run also selects the
+    128-wide GEMM tiles for the large products (ROCBTA_GEMM_TILE=128)."""
     for var in ("ROCBTA_POBTAF_ALGO", "ROCBTA_POBTAS_ALGO"):
         if request.param == "loop":
             monkeypatch.setenv(var, "loop")
         else:
             monkeypatch.delenv(var, raising=False)
+    if request.param == "loop":
+        monkeypatch.setenv("ROCBTA_GEMM_TILE", "128")
+    else:
+        monkeypatch.delenv("ROCBTA_GEMM_TILE", raising=False)
     return request.param
 
 
