@@ -49,6 +49,10 @@ struct GemmProblem {
   int uplo;    // TRI_NONE: full C; TRI_LOWER: only row >= col is written
   int mirror;  // with uplo == TRI_LOWER: also write C[col][row] = C[row][col]
   int tiles_m, tiles_n;
+  // lower_packed (uplo == TRI_LOWER, tiles_m == tiles_n): a batch entry has
+  // tiles_m (tiles_m + 1) / 2 workgroups, one per lower tile, instead of
+  // tiles_m * tiles_n of which the upper ones would exit at once.
+  int lower_packed;
   int tile_begin;  // first global tile index of this problem
   // Split-K: ksplit workgroups share one output tile, each over a contiguous
   // part of the K stages; partial tiles go to `partial` (ksplit * 4096
@@ -113,7 +117,7 @@ __device__ __forceinline__ void slice_load(const double* base, int64_t ld, int r
                                            int rows_ok, int cols_ok, double alpha,
                                            StageRegs& r) {
   constexpr int TPR = KCONT ? GTile<TS>::K / 2 : TS / 2;  // threads per stored row
-  const int tid = threadIdx.x;
+  const int tid = threadIdx.x & (GT_THREADS - 1);
   const int rsub = tid / TPR;
   const int c = (tid % TPR) * 2;
   constexpr int rstep = GT_THREADS / TPR;
@@ -135,7 +139,7 @@ __device__ __forceinline__ void slice_load(const double* base, int64_t ld, int r
 template <int TS, bool KCONT>
 __device__ __forceinline__ void slice_store(const StageRegs& r, double* X) {
   constexpr int TPR = KCONT ? GTile<TS>::K / 2 : TS / 2;
-  const int tid = threadIdx.x;
+  const int tid = threadIdx.x & (GT_THREADS - 1);
   const int rsub = tid / TPR;
   const int c = (tid % TPR) * 2;
   constexpr int rstep = GT_THREADS / TPR;
@@ -203,7 +207,7 @@ __device__ __forceinline__ void gemm_mfma_stage(const double* XA, const double* 
                                                 ACC (&acc)[GTile<TS>::NB][GTile<TS>::NB]) {
   constexpr int NB = GTile<TS>::NB;
   const int lane = threadIdx.x & 63;
-  const int wave = threadIdx.x >> 6;
+  const int wave = (threadIdx.x >> 6) & 3;
   const int wm = (wave >> 1) * (TS / 2), wn = (wave & 1) * (TS / 2);
   const int li = lane & 15, lk = lane >> 4;
 #pragma unroll
@@ -223,7 +227,13 @@ __device__ __forceinline__ void gemm_mfma_stage(const double* XA, const double* 
 // acc += alpha * op(A) op(B) over the K range [kb, ke): LDS double buffer
 // (slots 0/1) and two register stages in flight (x/y), unrolled by two so
 // that every register set has a fixed role.
-template <int TS, bool TA, bool TB, bool FAST, typename ACC>
+//
+// NH = 2 (512-thread workgroups): two 256-thread halves split the stages,
+// half h taking stages h, h + 2, h + 4, ... with its own LDS double buffer.
+// Both halves run the same number of iterations (a half without a stage in
+// the last one only passes the barriers), so every __syncthreads is reached
+// by all 512 threads.  Their accumulators are summed in gemm_tile_acc.
+template <int TS, int NH, bool TA, bool TB, bool FAST, typename ACC>
 __device__ __forceinline__ void gemm_term_pipeline(const TermView<TS, TA, TB, FAST>& v, int kb,
                                                    double* lds,
                                                    ACC (&acc)[GTile<TS>::NB][GTile<TS>::NB]) {
@@ -231,35 +241,39 @@ __device__ __forceinline__ void gemm_term_pipeline(const TermView<TS, TA, TB, FA
   const int ke = v.ke;
   const int nst = kb < ke ? (ke - kb + KS - 1) / KS : 0;
   if (nst == 0) return;
-  double* A0 = lds;
-  double* B0 = lds + SL;
-  double* A1 = lds + 2 * SL;
-  double* B1 = lds + 3 * SL;
+  const int h = (NH == 2) ? (int)(threadIdx.x >> 8) : 0;
+  const int nit = (nst + NH - 1) / NH;  // iterations, the same in both halves
+  auto has = [&](int it) { return it < nit && NH * it + h < nst; };
+  auto kof = [&](int it) { return kb + (NH * it + h) * KS; };
+  double* A0 = lds + h * 4 * SL;
+  double* B0 = A0 + SL;
+  double* A1 = A0 + 2 * SL;
+  double* B1 = A0 + 3 * SL;
   StageRegs xa, xb, ya, yb;
   __syncthreads();  // LDS free (previous term / previous use)
-  v.load(kb, xa, xb);
-  if (nst > 1) v.load(kb + KS, ya, yb);
-  v.store(xa, xb, A0, B0);
+  if (has(0)) v.load(kof(0), xa, xb);
+  if (has(1)) v.load(kof(1), ya, yb);
+  if (has(0)) v.store(xa, xb, A0, B0);
   __syncthreads();
-  if (nst > 2) v.load(kb + 2 * KS, xa, xb);
-  for (int s = 0; s < nst; s += 2) {
-    // slot 0 holds stage s; y holds stage s+1; x holds stage s+2 (in flight)
-    gemm_mfma_stage<TS, TA, TB, ACC>(A0, B0, acc);
-    if (s + 1 >= nst) break;
-    v.store(ya, yb, A1, B1);
+  if (has(2)) v.load(kof(2), xa, xb);
+  for (int it = 0; it < nit; it += 2) {
+    // slot 0 holds iteration it; y holds it+1; x holds it+2 (in flight)
+    if (has(it)) gemm_mfma_stage<TS, TA, TB, ACC>(A0, B0, acc);
+    if (it + 1 >= nit) break;
+    if (has(it + 1)) v.store(ya, yb, A1, B1);
     __syncthreads();
-    if (s + 3 < nst) v.load(kb + (s + 3) * KS, ya, yb);
-    gemm_mfma_stage<TS, TA, TB, ACC>(A1, B1, acc);
-    if (s + 2 >= nst) break;
-    v.store(xa, xb, A0, B0);
+    if (has(it + 3)) v.load(kof(it + 3), ya, yb);
+    if (has(it + 1)) gemm_mfma_stage<TS, TA, TB, ACC>(A1, B1, acc);
+    if (it + 2 >= nit) break;
+    if (has(it + 2)) v.store(xa, xb, A0, B0);
     __syncthreads();
-    if (s + 4 < nst) v.load(kb + (s + 4) * KS, xa, xb);
+    if (has(it + 4)) v.load(kof(it + 4), xa, xb);
   }
 }
 
 // Stages [sb, se) of the term (a stage = K consecutive k from the term's
 // first k); the K range of those stages is clipped to the term's range.
-template <int TS, bool TA, bool TB, typename ACC>
+template <int TS, int NH, bool TA, bool TB, typename ACC>
 __device__ __forceinline__ void gemm_term(const GemmTerm& t, const double* A, const double* B,
                                           int M, int N, int m0, int n0, int sb, int se,
                                           double* lds, ACC (&acc)[GTile<TS>::NB][GTile<TS>::NB]) {
@@ -273,35 +287,45 @@ __device__ __forceinline__ void gemm_term(const GemmTerm& t, const double* A, co
                     n0 + TS <= N && ((ke - kb) % KS) == 0;
   if (fast) {
     TermView<TS, TA, TB, true> v{A, B, t.lda, t.ldb, M, N, m0, n0, ke, t.alpha};
-    gemm_term_pipeline<TS, TA, TB, true, ACC>(v, kb, lds, acc);
+    gemm_term_pipeline<TS, NH, TA, TB, true, ACC>(v, kb, lds, acc);
   } else {
     TermView<TS, TA, TB, false> v{A, B, t.lda, t.ldb, M, N, m0, n0, ke, t.alpha};
-    gemm_term_pipeline<TS, TA, TB, false, ACC>(v, kb, lds, acc);
+    gemm_term_pipeline<TS, NH, TA, TB, false, ACC>(v, kb, lds, acc);
   }
 }
 
 // One TS x TS output tile (global tile index `tile`) of problem P, with
 // accumulators of type ACC (f64x4: FP64, f32x4: mixed precision).  The
 // problem's tiles_m / tiles_n / tile_begin were computed for this TS.
-template <int TS, typename ACC>
+template <int TS, int NH, typename ACC>
 __device__ __forceinline__ void gemm_tile_acc(const GemmProblem& P, int tile, double* lds) {
   constexpr int NB = GTile<TS>::NB, KS = GTile<TS>::K;
   int local = (tile - P.tile_begin);
   const int split = local % P.ksplit;
   local /= P.ksplit;
   const int tile_id = local;  // tile index within the problem (for split-K bookkeeping)
-  const int per_batch = P.tiles_m * P.tiles_n;
+  const int per_batch =
+      P.lower_packed ? P.tiles_m * (P.tiles_m + 1) / 2 : P.tiles_m * P.tiles_n;
   const int bidx = local / per_batch;
   local -= bidx * per_batch;
   const int b1 = bidx % P.batch1, b2 = bidx / P.batch1;
-  const int tm = local / P.tiles_n, tn = local % P.tiles_n;
+  int tm, tn;
+  if (P.lower_packed) {  // local = tm (tm + 1) / 2 + tn, 0 <= tn <= tm
+    tm = (int)((__builtin_sqrtf(8.0f * (float)local + 1.0f) - 1.0f) * 0.5f);
+    while ((tm + 1) * (tm + 2) / 2 <= local) ++tm;
+    while (tm * (tm + 1) / 2 > local) --tm;
+    tn = local - tm * (tm + 1) / 2;
+  } else {
+    tm = local / P.tiles_n;
+    tn = local % P.tiles_n;
+  }
   const int m0 = tm * TS, n0 = tn * TS;
   if (P.uplo == TRI_LOWER && n0 > m0 + TS - 1) return;
   if (P.uplo == TRI_UPPER && m0 > n0 + TS - 1) return;
 
   double* C = P.C + b1 * P.sC1 + b2 * P.sC2;
   const int lane = threadIdx.x & 63;
-  const int wave = threadIdx.x >> 6;
+  const int wave = (threadIdx.x >> 6) & 3;
   const int wm = (wave >> 1) * (TS / 2), wn = (wave & 1) * (TS / 2);
 
   ACC acc[NB][NB];
@@ -328,29 +352,59 @@ __device__ __forceinline__ void gemm_tile_acc(const GemmProblem& P, int tile, do
     const double* A = t.A + b1 * t.sA1 + b2 * t.sA2;
     const double* B = t.B + b1 * t.sB1 + b2 * t.sB2;
     if (!t.transA && !t.transB)
-      gemm_term<TS, false, false, ACC>(t, A, B, P.M, P.N, m0, n0, sb, se, lds, acc);
+      gemm_term<TS, NH, false, false, ACC>(t, A, B, P.M, P.N, m0, n0, sb, se, lds, acc);
     else if (!t.transA && t.transB)
-      gemm_term<TS, false, true, ACC>(t, A, B, P.M, P.N, m0, n0, sb, se, lds, acc);
+      gemm_term<TS, NH, false, true, ACC>(t, A, B, P.M, P.N, m0, n0, sb, se, lds, acc);
     else if (t.transA && !t.transB)
-      gemm_term<TS, true, false, ACC>(t, A, B, P.M, P.N, m0, n0, sb, se, lds, acc);
+      gemm_term<TS, NH, true, false, ACC>(t, A, B, P.M, P.N, m0, n0, sb, se, lds, acc);
     else
-      gemm_term<TS, true, true, ACC>(t, A, B, P.M, P.N, m0, n0, sb, se, lds, acc);
+      gemm_term<TS, NH, true, true, ACC>(t, A, B, P.M, P.N, m0, n0, sb, se, lds, acc);
+  }
+
+  // NH = 2: the upper half adds its accumulators to the lower half's
+  // through LDS; the lower half (threads 0..255) runs the epilogue, the upper
+  // half only passes the remaining barriers.
+  const bool lead = (NH == 1) || threadIdx.x < GT_THREADS;
+  if (NH == 2) {
+    __syncthreads();  // both pipelines are done with LDS
+    const int tl = threadIdx.x & (GT_THREADS - 1);
+    if (!lead) {
+#pragma unroll
+      for (int i = 0; i < NB; ++i)
+#pragma unroll
+        for (int j = 0; j < NB; ++j)
+#pragma unroll
+          for (int r = 0; r < 4; ++r) lds[((i * NB + j) * 4 + r) * GT_THREADS + tl] = acc[i][j][r];
+    }
+    __syncthreads();
+    if (lead) {
+#pragma unroll
+      for (int i = 0; i < NB; ++i)
+#pragma unroll
+        for (int j = 0; j < NB; ++j)
+#pragma unroll
+          for (int r = 0; r < 4; ++r) acc[i][j][r] += lds[((i * NB + j) * 4 + r) * GT_THREADS + tl];
+    }
   }
 
   if (P.ksplit > 1) {
     // Publish this split's partial tile; the last finisher sums all of them
     // in split order (deterministic) and runs the epilogue.
-    const int tid = threadIdx.x;
+    const int tid = threadIdx.x & (GT_THREADS - 1);
     double* mine = P.partial + ((int64_t)tile_id * P.ksplit + split) * (TS * TS);
 #pragma unroll
     for (int i = 0; i < NB; ++i)
 #pragma unroll
       for (int j = 0; j < NB; ++j)
 #pragma unroll
-        for (int r = 0; r < 4; ++r) mine[((i * NB + j) * 4 + r) * GT_THREADS + tid] = acc[i][j][r];
+        for (int r = 0; r < 4; ++r)
+          if (lead) mine[((i * NB + j) * 4 + r) * GT_THREADS + tid] = acc[i][j][r];
+    // every wave releases its own partial stores before the barrier (the
+    // barrier does not wait for them; thread 0's release covers its wave)
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
     __syncthreads();
     __shared__ int s_last;
-    if (tid == 0) {
+    if (threadIdx.x == 0) {
       const int old = __hip_atomic_fetch_add(&P.counters[tile_id], 1, __ATOMIC_ACQ_REL,
                                              __HIP_MEMORY_SCOPE_AGENT);
       s_last = (old == P.ksplit - 1);
@@ -358,7 +412,7 @@ __device__ __forceinline__ void gemm_tile_acc(const GemmProblem& P, int tile, do
                                      __HIP_MEMORY_SCOPE_AGENT);
     }
     __syncthreads();
-    if (!s_last) return;
+    if (!s_last || !lead) return;
     const double* base = P.partial + (int64_t)tile_id * P.ksplit * (TS * TS);
 #pragma unroll
     for (int i = 0; i < NB; ++i)
@@ -374,6 +428,7 @@ __device__ __forceinline__ void gemm_tile_acc(const GemmProblem& P, int tile, do
         }
   }
 
+  if (!lead) return;
 #pragma unroll
   for (int i = 0; i < NB; ++i)
 #pragma unroll
@@ -393,27 +448,52 @@ __device__ __forceinline__ void gemm_tile_acc(const GemmProblem& P, int tile, do
       }
 }
 
-template <int TS>
+template <int TS, int NH = 1>
 __device__ __forceinline__ void gemm_tile(const GemmProblem& P, int tile, double* lds) {
   if (P.f32)
-    gemm_tile_acc<TS, f32x4>(P, tile, lds);
+    gemm_tile_acc<TS, NH, f32x4>(P, tile, lds);
   else
-    gemm_tile_acc<TS, f64x4>(P, tile, lds);
+    gemm_tile_acc<TS, NH, f64x4>(P, tile, lds);
+}
+
+// Workgroup -> tile order.  Workgroups are dealt to the 8 XCDs round-robin
+// (workgroup w runs on XCD w mod 8), each with its own L2.  Giving XCD x a
+// contiguous range of tile indices (row-major within a problem: whole tile
+// rows) makes the workgroups of one XCD share their A row panels in that
+// L2 instead of every XCD streaming every panel.  A bijection of
+// [0, ntiles), so any dispatch order stays correct.
+constexpr int N_XCD = 8;
+__device__ __forceinline__ int xcd_tile(int w, int ntiles) {
+  const int q = ntiles / N_XCD, r = ntiles % N_XCD;
+  const int x = w % N_XCD, i = w / N_XCD;
+  return x * q + min(x, r) + i;
 }
 
 __global__ void __launch_bounds__(GT_THREADS, 2) gemm_group_kernel(GemmGroup g) {
   __shared__ __attribute__((aligned(16))) double lds[4 * GTile<64>::SLICE];
-  const int tile = blockIdx.x;
+  const int tile = xcd_tile(blockIdx.x, g.total_tiles);
   int pi = 0;
   while (pi + 1 < g.nprob && tile >= g.p[pi + 1].tile_begin) ++pi;
   gemm_tile<64>(g.p[pi], tile, lds);
+}
+
+// 64 x 64 tiles with 512 threads (one workgroup per CU, two waves per SIMD):
+// the K stages of a tile are split between two 256-thread halves.  For
+// launches with about one tile per CU this doubles the waves that issue
+// MFMAs without a split-K pass through global memory.
+__global__ void __launch_bounds__(2 * GT_THREADS, 1) gemm_group_kernel_h2(GemmGroup g) {
+  __shared__ __attribute__((aligned(16))) double lds[8 * GTile<64>::SLICE];
+  const int tile = xcd_tile(blockIdx.x, g.total_tiles);
+  int pi = 0;
+  while (pi + 1 < g.nprob && tile >= g.p[pi + 1].tile_begin) ++pi;
+  gemm_tile<64, 2>(g.p[pi], tile, lds);
 }
 
 // 128 x 128 tiles for large products: one workgroup per CU, so that each
 // wave can hold its sixteen accumulators without spilling.
 __global__ void __launch_bounds__(GT_THREADS, 1) gemm_group_kernel_t128(GemmGroup g) {
   __shared__ __attribute__((aligned(16))) double lds[4 * GTile<128>::SLICE];
-  const int tile = blockIdx.x;
+  const int tile = xcd_tile(blockIdx.x, g.total_tiles);
   int pi = 0;
   while (pi + 1 < g.nprob && tile >= g.p[pi + 1].tile_begin) ++pi;
   gemm_tile<128>(g.p[pi], tile, lds);

@@ -246,17 +246,23 @@ __device__ __forceinline__ void chain_wait(const PobtafChain& P, const int* fa, 
   __syncthreads();
 }
 
+// Every wave waits for its own global stores to complete (s_waitcnt
+// vmcnt(0)) before the barrier: a workgroup barrier does not wait for them,
+// and the release by thread 0 (L2 write-back, then its own wave's counter)
+// would otherwise miss stores of other waves still on their way to L2.
+// With all of them in L2, thread 0's single release covers the tile.
+__device__ __forceinline__ void wave_stores_done() { __builtin_amdgcn_s_waitcnt(0x0F70); }
+
+// The explicit wait after thread 0's release fence keeps the flag store
+// behind the completion of the L2 write-back: with the counters already
+// drained, the compiler otherwise emits no wait between the two.
 __device__ __forceinline__ void chain_publish(int* f) {
-  __syncthreads();  // all waves' stores of the tile have completed
-  if (threadIdx.x == 0) __hip_atomic_store(f, 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-}
-// Two flags behind one release.
-__device__ __forceinline__ void chain_publish2(int* f1, int* f2) {
+  wave_stores_done();
   __syncthreads();
   if (threadIdx.x == 0) {
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
-    __hip_atomic_store(f1, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    __hip_atomic_store(f2, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    wave_stores_done();
+    __hip_atomic_store(f, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   }
 }
 
@@ -488,6 +494,10 @@ __device__ void chain_run(const PobtafChain& P, double* S, double* Xs, double* W
   const int lane = tid & 63, wave = tid >> 6;
   const int wm = (wave >> 1) * 32, wn = (wave & 1) * 32;
   const int li = lane & 15, lk = lane >> 4;
+  // Linv tiles keep zeros above their 16x16 block diagonal from here on
+  // (full-tile inversions never write there; ragged ones clear it).
+  for (int idx = tid; idx < CT * TT_LD; idx += CH_THREADS) Xs[idx] = 0.0;
+  __syncthreads();
   for (int g = 0; g < P.NT; ++g) {
     const CTile dt = tile_g(P, g, g);
     f64x4 acc[2][2];
@@ -535,16 +545,14 @@ __device__ void chain_run(const PobtafChain& P, double* S, double* Xs, double* W
       const int rr = idx >> 6, cc = idx & 63;
       if (rr < m && cc < m) dt.p[(int64_t)rr * dt.ld + cc] = (cc <= rr) ? S2[rr][cc] : 0.0;
     }
-    lds_trtri(S2, X2, W2, m);
+    lds_trtri<false>(S2, X2, W2, m);
     double* li_out = P.linv + (int64_t)g * CT * CT;
     for (int idx = tid; idx < CT * CT; idx += CH_THREADS) li_out[idx] = X2[idx >> 6][idx & 63];
-    if (g + 1 >= band_end(P, g)) {
-      chain_publish(P.cflags + 2 * g);
-      continue;
-    }
-    // L(g,g) and Linv(g) are published together with L(g+1,g) at the end of
-    // the step: one release instead of two on the chain's path (no helper
-    // task feeding PX(g) needs Linv(g)).
+    // Linv(g) is published before the sub-diagonal solve: the helpers'
+    // solves of column g (TRSM(g+2,g) feeds PX(g+1), which the next step
+    // waits for) start while the chain finishes this step.
+    chain_publish(P.cflags + 2 * g);
+    if (g + 1 >= band_end(P, g)) continue;
     // L(g+1,g) = PX(g) * Linv(g)^T
     chain_wait(P, P.hflags + t_px(P, g), nullptr);
     const CTile xt = tile_g(P, g + 1, g);
@@ -592,7 +600,7 @@ __device__ void chain_run(const PobtafChain& P, double* S, double* Xs, double* W
     cacc_store(xt, acc);
     __syncthreads();  // S no longer read as X
     cacc_to_lds(acc, S, TT_LD);  // keep L(g+1,g) as S[row][k]
-    chain_publish2(P.cflags + 2 * g, P.cflags + 2 * g + 1);
+    chain_publish(P.cflags + 2 * g + 1);
   }
 }
 
@@ -671,10 +679,12 @@ __device__ __forceinline__ void chain_helper_task(const PobtafChain& P, int task
   helper_accumulate<TAG>(P, cr, -1.0, kind == HK_PD, XA, XB, acc);
   if (kind == HK_TRSM || kind == HK_ARR) helper_solve<TAG>(P, j, XA, XB, acc);
   cacc_store(out, acc);
-  // the strict-upper mirror of a lower D-block tile is zero in the factor
+  chain_publish(P.hflags + task);
+  // The strict-upper mirror of a lower D-block tile is zero in the factor.
+  // No task reads it, so it is cleared after the publish, off the path of
+  // the tasks waiting for this one.
   if (kind == HK_TRSM && r / P.nt == j / P.nt) tile_zero(tile_mirror(P, r, j));
   if (kind == HK_PX && (j + 1) / P.nt == j / P.nt) tile_zero(tile_mirror(P, j + 1, j));
-  chain_publish(P.hflags + task);
 }
 
 __global__ void __launch_bounds__(CH_THREADS) pobtaf_chain_kernel(PobtafChain P) {

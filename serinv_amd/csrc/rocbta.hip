@@ -165,12 +165,18 @@ int run_gemms(GemmList& list, hipStream_t s) {
     return ROCBTA_OK;
   }
   const int ts = gemm_ts(list);
+  static const bool h2 = [] {
+    const char* e = getenv("ROCBTA_GEMM_H2");
+    return e && strcmp(e, "1") == 0;
+  }();
   GemmGroup g{};
   auto flush = [&]() -> int {
     if (g.nprob == 0) return ROCBTA_OK;
     if (g.total_tiles > 0) {
       if (ts == 128)
         gemm_group_kernel_t128<<<g.total_tiles, GT_THREADS, 0, s>>>(g);
+      else if (h2)
+        gemm_group_kernel_h2<<<g.total_tiles, 2 * GT_THREADS, 0, s>>>(g);
       else
         gemm_group_kernel<<<g.total_tiles, GT_THREADS, 0, s>>>(g);
       RB_CHECK(hipGetLastError());
@@ -185,7 +191,10 @@ int run_gemms(GemmList& list, hipStream_t s) {
     p.tiles_n = cdiv(p.N, ts);
     p.f32 = g_mixed ? 1 : 0;
     if (p.ksplit < 1 || !p.partial || !p.counters) p.ksplit = 1;
-    const int64_t tiles = (int64_t)p.tiles_m * p.tiles_n * p.batch1 * p.batch2 * p.ksplit;
+    p.lower_packed = (p.uplo == TRI_LOWER && p.tiles_m == p.tiles_n) ? 1 : 0;
+    const int64_t per_batch =
+        p.lower_packed ? (int64_t)p.tiles_m * (p.tiles_m + 1) / 2 : (int64_t)p.tiles_m * p.tiles_n;
+    const int64_t tiles = per_batch * p.batch1 * p.batch2 * p.ksplit;
     if (g.nprob == ROCBTA_MAX_PROBLEMS || (int64_t)g.total_tiles + tiles > (1ll << 30)) {
       int rc = flush();
       if (rc) return rc;
@@ -313,7 +322,7 @@ int factor_block_column(double* Dm, int64_t ld, int m, const std::vector<Panel>&
 // row stride lds) into X (row stride ldx, stride sX), upper part zeroed.
 // W: scratch of batch * ceil(m/64) * 64 * 64 doubles.
 int block_trtri(const double* Ls, int64_t lds, int64_t sL, double* X, int64_t ldx, int64_t sX,
-                int m, int batch, double* W, hipStream_t s) {
+                int m, int batch, double* W, hipStream_t s, bool halving = false) {
   if (m <= 0 || batch <= 0) return ROCBTA_OK;
   const int nt = cdiv(m, TT);
   const int nfull = m / TT;
@@ -329,6 +338,31 @@ int block_trtri(const double* Ls, int64_t lds, int64_t sL, double* X, int64_t ld
                      X + (int64_t)nfull * TT * (ldx + 1), ldx, 0, sX, tail, 1, batch, nullptr, 0,
                      0, 0, 0, s);
     if (rc) return rc;
+  }
+  if (halving && tail == 0 && (nt & (nt - 1)) == 0) {
+    // Recursive halving, bottom-up: with L = [A 0; C B] (halves of order h),
+    //   L^{-1} = [A^{-1} 0; -B^{-1} (C A^{-1}) B^{-1}],
+    // so level h forms W = C A^{-1}, then X21 = -B^{-1} W, for all m / 2h
+    // pairs of every matrix in two batched launches.  log2(nt) levels of
+    // large products replace nt - 1 levels of 64-wide ones.  W holds m^2 / 4
+    // doubles per matrix.
+    const int64_t wstride = (int64_t)m * m / 4;
+    for (int h = TT; h < m; h *= 2) {
+      const int npair = m / (2 * h);
+      const int64_t pL = (int64_t)2 * h * (lds + 1), pX = (int64_t)2 * h * (ldx + 1);
+      GemmList w, x;
+      push_term(w.add(W, h, h, h, 0.0, TRI_NONE, npair, (int64_t)h * h, batch, wstride),
+                term(Ls + (int64_t)h * lds, lds, false, X, ldx, false, h, 1.0, TRI_NONE, TRI_LOWER,
+                     pL, pX, sL, sX));
+      push_term(x.add(X + (int64_t)h * ldx, ldx, h, h, 0.0, TRI_NONE, npair, pX, batch, sX),
+                term(X + (int64_t)h * (ldx + 1), ldx, false, W, h, false, h, -1.0, TRI_LOWER,
+                     TRI_NONE, pX, (int64_t)h * h, sX, wstride));
+      rc = run_gemms(w, s);
+      if (rc) return rc;
+      rc = run_gemms(x, s);
+      if (rc) return rc;
+    }
+    return ROCBTA_OK;
   }
   const int64_t wstride = (int64_t)nt * TT * TT;
   for (int d = 1; d < nt; ++d) {
@@ -404,7 +438,8 @@ SiLayout si_layout(int64_t n, int64_t b, int64_t a) {
   l.linv = take((size_t)nch * b * b);
   l.g = take((size_t)std::max<int64_t>(n - 1, 0) * b * b);
   l.gt = take((size_t)n * a * b);
-  l.w = take((size_t)nch * ntb * TT * TT);
+  // trtri scratch: 64-wide levels need ntb * 64^2 per block, halving b^2 / 4
+  l.w = take((size_t)nch * std::max<int64_t>(ntb * TT * TT, ntb * TT * ntb * TT / 4));
   l.tinv = take((size_t)a * a);
   l.wt = take((size_t)nta * TT * TT);
   // split-K partial tiles: up to 2 splits of 64-tiles or 8 splits of 128-tiles
@@ -753,11 +788,10 @@ int rocbta_dpobtasi(int64_t n, int64_t b, int64_t a, double* D, double* L, doubl
   const int ts = gemm_ts_for((int)b, (int)b, (int)b);
   const int nts = cdiv(b, ts);
   const int cus = device_cus();
+  // 64-tiles: no split (one 1024^2 product is 256 tiles, one per CU, and a
+  // split through global partials measured slower than none).
   int ksplit = 1;
-  if (ts == 128)
-    ksplit = std::max(1, std::min(8, cus / (nts * nts)));
-  else if (nts * nts < 2 * cus && b >= 256)
-    ksplit = 2;
+  if (ts == 128) ksplit = std::max(1, std::min(8, cus / (nts * nts)));
   if (ksplit > 1) {
     fill_i32_kernel<<<1, 256, 0, s>>>(cnt, ntb * ntb + ntb, 0);
     RB_CHECK(hipGetLastError());
@@ -788,7 +822,7 @@ int rocbta_dpobtasi(int64_t n, int64_t b, int64_t a, double* D, double* L, doubl
     const int hi = npre - k * SI_CHUNK;             // exclusive
     const int lo = std::max(0, hi - SI_CHUNK);      // inclusive
     const int nblk = hi - lo;
-    int rc = block_trtri(D + lo * bb, b, bb, Linv, b, bb, bi, nblk, W, ps);
+    int rc = block_trtri(D + lo * bb, b, bb, Linv, b, bb, bi, nblk, W, ps, true);
     if (rc) return rc;
     GemmList pre;
     const int ng = std::min<int>(hi, (int)n - 1) - lo;
@@ -843,7 +877,27 @@ int rocbta_dpobtasi(int64_t n, int64_t b, int64_t a, double* D, double* L, doubl
       if (rc) return rc;
     }
   }
-  // (3) backward recurrence (pobtasi.py:141-172) with the precomputed G's
+  // (3) backward recurrence (pobtasi.py:141-172) with the precomputed G's.
+  // The arrow row block X_{t,i} = -X_{t,i+1} G_i - X_T Gt_i depends on
+  // neither X_{i+1,i} nor X_ii, so it rides in the launch of X_{i+1,i+1}
+  // (whose upper tiles leave CUs idle) instead of in the launch of
+  // X_{i+1,i}, which then is exactly one 64-tile per CU at b = 1024.
+  auto arrow_row = [&](GemmList& gl, int64_t i) {
+    GemmProblem& pr = gl.add(R + i * ab, b, ai, bi, 0.0);
+    pr.ksplit = ksplit;
+    pr.partial = part_r;
+    pr.counters = cnt_r;
+    push_term(pr, term(R + (i + 1) * ab, b, false, G + i * bb, b, false, bi, -1.0));
+    push_term(pr, term(T, a, false, Gt + i * ab, b, false, ai, -1.0));
+  };
+  if (a > 0 && n >= 2) {
+    rc = need(n - 2);
+    if (rc) return rc;
+    GemmList first;
+    arrow_row(first, n - 2);
+    rc = run_gemms(first, s);
+    if (rc) return rc;
+  }
   for (int64_t i = n - 2; i >= 0; --i) {
     rc = need(i);
     if (rc) return rc;
@@ -853,15 +907,7 @@ int rocbta_dpobtasi(int64_t n, int64_t b, int64_t a, double* D, double* L, doubl
     pl.partial = part;
     pl.counters = cnt;
     push_term(pl, term(D + (i + 1) * bb, b, false, G + i * bb, b, false, bi, -1.0));
-    if (a > 0) {
-      push_term(pl, term(R + (i + 1) * ab, b, true, Gt + i * ab, b, false, ai, -1.0));
-      GemmProblem& pr = off.add(R + i * ab, b, ai, bi, 0.0);
-      pr.ksplit = ksplit;
-      pr.partial = part_r;
-      pr.counters = cnt_r;
-      push_term(pr, term(R + (i + 1) * ab, b, false, G + i * bb, b, false, bi, -1.0));
-      push_term(pr, term(T, a, false, Gt + i * ab, b, false, ai, -1.0));
-    }
+    if (a > 0) push_term(pl, term(R + (i + 1) * ab, b, true, Gt + i * ab, b, false, ai, -1.0));
     rc = run_gemms(off, s);
     if (rc) return rc;
     // X_ii = S_i - X_{i+1,i}^T G_i - X_{t,i}^T Gt_i is symmetric: lower
@@ -874,6 +920,11 @@ int rocbta_dpobtasi(int64_t n, int64_t b, int64_t a, double* D, double* L, doubl
     pd.counters = cnt;
     push_term(pd, term(L + i * bb, b, true, G + i * bb, b, false, bi, -1.0));
     if (a > 0) push_term(pd, term(R + i * ab, b, true, Gt + i * ab, b, false, ai, -1.0));
+    if (a > 0 && i >= 1) {
+      rc = need(i - 1);
+      if (rc) return rc;
+      arrow_row(dia, i - 1);
+    }
     rc = run_gemms(dia, s);
     if (rc) return rc;
   }

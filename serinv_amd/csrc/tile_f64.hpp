@@ -104,6 +104,62 @@ __device__ __forceinline__ void wave_panel_potrf(double (*S)[TT_LD], int p, int 
     if (act && k < w && p + k <= i) S[i][p + k] = r[k];
 }
 
+// Full-width (16-column) form of the panel Cholesky, p + 16 <= m.  The wave
+// is bound by instruction issue, and most of the issue went into readlane
+// broadcasts of the row entries L(k,j) (two per double).  Here the panel is
+// cut into four 4-column sub-panels:
+//   * inside a sub-panel, right-looking with readlane broadcasts (at most
+//     3 values per pivot);
+//   * a finished sub-panel is written to S; the rows k of later sub-panels
+//     read their four entries L(k, c0..c0+3) back as LDS broadcasts (one
+//     address for all lanes), except the next pivot column c0+4, which
+//     takes readlanes because it is on the dependency path.
+// A non-positive or NaN pivot turns into NaN through the reciprocal square
+// root, so the loop has no branch; the first failing column is recorded
+// once after it.  Lanes above the diagonal of a column hold values that
+// are never read or written back.
+__device__ __forceinline__ void wave_panel_potrf16(double (*S)[TT_LD], int p, int m, int* info,
+                                                   int64_t row0) {
+  const int lane = threadIdx.x & 63;
+  const int i = p + lane;
+  const bool act = i < m;
+  const int ic = act ? i : p;  // clamped row for the loads
+  double r[PW];
+#pragma unroll
+  for (int k = 0; k < PW; ++k) r[k] = act ? S[ic][p + k] : 0.0;
+  int bad = PW;
+#pragma unroll
+  for (int c0 = 0; c0 < PW; c0 += 4) {
+#pragma unroll
+    for (int jj = 0; jj < 4; ++jj) {
+      const int j = c0 + jj;
+      const double piv = readlane_f64(r[j], j);
+      bad = (bad == PW && !(piv > 0.0)) ? j : bad;
+      double d, inv;
+      pivot_sqrt(piv, d, inv);
+      r[j] = (lane == j) ? d : r[j] * inv;
+#pragma unroll
+      for (int k = j + 1; k < c0 + 4; ++k) r[k] -= r[j] * readlane_f64(r[j], k);
+    }
+#pragma unroll
+    for (int q = 0; q < 4; ++q)
+      if (act && p + c0 + q <= i) S[i][p + c0 + q] = r[c0 + q];
+    if (c0 + 4 < PW) {
+      __builtin_amdgcn_wave_barrier();  // the sub-panel's LDS stores precede the reads below
+#pragma unroll
+      for (int q = 0; q < 4; ++q) r[c0 + 4] -= r[c0 + q] * readlane_f64(r[c0 + q], c0 + 4);
+#pragma unroll
+      for (int k = c0 + 5; k < PW; ++k) {
+        const double* row = &S[p + k][p + c0];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) r[k] -= r[c0 + q] * row[q];
+      }
+      __builtin_amdgcn_wave_barrier();
+    }
+  }
+  if (bad < PW && lane == 0) record_info(info, (int)(row0 + p + bad + 1));
+}
+
 // S[q:m, q:m] (lower part) -= P P^T with P = S[q:m, p:p+w], on MFMA, all
 // waves.  Output 16x16 blocks are dealt round-robin to the waves.
 template <int WFIX>
@@ -146,8 +202,8 @@ __device__ __forceinline__ void lds_potrf_t(double (*S)[TT_LD], int m_rt, int* i
 #pragma unroll
   for (int p = 0; p < m; p += PW) {
     const int w = min(PW, m - p);
-    if (MFIX) {
-      if (wave == 0) wave_panel_potrf<PW>(S, p, m, PW, info, row0);
+    if (MFIX || w == PW) {
+      if (wave == 0) wave_panel_potrf16(S, p, m, info, row0);
     } else {
       if (wave == 0) wave_panel_potrf<0>(S, p, m, w, info, row0);
     }
@@ -177,15 +233,48 @@ __device__ __forceinline__ void lds_potrf(double (*S)[TT_LD], int m, int* info, 
 // Ws: 64 x 17 scratch; at distance d the block (I, I-d) of W sits in rows
 // 16I .. 16I+15.
 constexpr int WS_LD = 17;
-template <int MFIX>
+
+// Inverse of the full 16x16 lower-triangular block of S at (o, o) by the
+// calling wave, written to Xs as a full block (zeros above the diagonal).
+// Lane c (and its copies c + 16, c + 32, c + 48) forms column c by forward
+// substitution, x_i = (delta_ic - sum_{k<i} L_ik x_k) / L_ii; the entries
+// L_ik are LDS broadcasts and 1 / L_ii comes from lane i by readlane, so the
+// step needs no division on its dependency path and no cross-lane shuffle.
+__device__ __forceinline__ void wave_diag_inv16(const double (*S)[TT_LD], double (*Xs)[TT_LD],
+                                                int o) {
+  const int lane = threadIdx.x & 63;
+  const int c = lane & 15;
+  const double dinv = 1.0 / S[o + c][o + c];
+  double x[16];
+#pragma unroll
+  for (int i = 0; i < 16; ++i) {
+    double acc = (i == c) ? 1.0 : 0.0;
+#pragma unroll
+    for (int k = 0; k < i; ++k) acc -= S[o + i][o + k] * x[k];
+    x[i] = acc * readlane_f64(dinv, i);
+  }
+  if (lane < 16) {
+#pragma unroll
+    for (int i = 0; i < 16; ++i) Xs[o + i][o + c] = x[i];
+  }
+}
+
+// ZERO = false: the caller guarantees that the blocks of Xs strictly above
+// the 16x16 block diagonal already hold zeros (they are never written here).
+template <int MFIX, bool ZERO = true>
 __device__ __forceinline__ void lds_trtri_t(const double (*S)[TT_LD], double (*Xs)[TT_LD],
                                             double (*Ws)[WS_LD], int m_rt) {
   const int m = MFIX ? MFIX : m_rt;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int tid = threadIdx.x;
-  for (int idx = tid; idx < TT * TT; idx += TT_THREADS) Xs[idx / TT][idx % TT] = 0.0;
-  __syncthreads();
+  if (ZERO || !MFIX) {
+    for (int idx = tid; idx < TT * TT; idx += TT_THREADS) Xs[idx / TT][idx % TT] = 0.0;
+    __syncthreads();
+  }
   const int nb = (m + 15) / 16;
+  if (MFIX == TT) {
+    wave_diag_inv16(S, Xs, __builtin_amdgcn_readfirstlane(16 * wave));
+  } else
   for (int blk = wave; blk < nb; blk += 4) {
     const int o = blk * 16, w = min(16, m - o);
     const int i = lane & 15, g = lane >> 4;
@@ -252,10 +341,11 @@ __device__ __forceinline__ void lds_trtri_t(const double (*S)[TT_LD], double (*X
   }
 }
 
+template <bool ZERO = true>
 __device__ __forceinline__ void lds_trtri(const double (*S)[TT_LD], double (*Xs)[TT_LD],
                                           double (*Ws)[WS_LD], int m) {
   if (m == TT)
-    lds_trtri_t<TT>(S, Xs, Ws, TT);
+    lds_trtri_t<TT, ZERO>(S, Xs, Ws, TT);
   else
     lds_trtri_t<0>(S, Xs, Ws, m);
 }
