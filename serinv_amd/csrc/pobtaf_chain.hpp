@@ -498,11 +498,24 @@ __device__ void chain_run(const PobtafChain& P, double* S, double* Xs, double* W
   // (full-tile inversions never write there; ragged ones clear it).
   for (int idx = tid; idx < CT * TT_LD; idx += CH_THREADS) Xs[idx] = 0.0;
   __syncthreads();
+  // PD(g+1) is usually ready when the chain waits for PX(g): it is waited
+  // for together with PX(g) and loaded into registers (pd_next) then, so
+  // that the next step starts without the load latency.
+  f64x4 pd_next[2][2];
+  bool have_pd = false;
   for (int g = 0; g < P.NT; ++g) {
     const CTile dt = tile_g(P, g, g);
     f64x4 acc[2][2];
-    if (g > 0) chain_wait(P, P.hflags + t_pd(P, g), nullptr);
-    cacc_load(dt, acc);
+    if (have_pd) {
+#pragma unroll
+      for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j) acc[i][j] = pd_next[i][j];
+    } else {
+      if (g > 0) chain_wait(P, P.hflags + t_pd(P, g), nullptr);
+      cacc_load(dt, acc);
+    }
+    have_pd = false;
     if (g > 0 && P.f32) {
       f32x4 a32[2][2];
       cacc32_zero(a32);
@@ -554,11 +567,16 @@ __device__ void chain_run(const PobtafChain& P, double* S, double* Xs, double* W
     chain_publish(P.cflags + 2 * g);
     if (g + 1 >= band_end(P, g)) continue;
     // L(g+1,g) = PX(g) * Linv(g)^T
-    chain_wait(P, P.hflags + t_px(P, g), nullptr);
+    const bool pre = g + 1 < P.NT;
+    chain_wait(P, P.hflags + t_px(P, g), pre ? P.hflags + t_pd(P, g + 1) : nullptr);
     const CTile xt = tile_g(P, g + 1, g);
     {
       TileRegs16 rx;
       tile_load16(xt, rx);
+      if (pre) {
+        cacc_load(tile_g(P, g + 1, g + 1), pd_next);
+        have_pd = true;
+      }
       // S[m][k] = X(m, k) (pitch 65 rows are 8-byte aligned only: scalar stores)
       const int rs = tid >> 5, c = (tid & 31) * 2;
 #pragma unroll

@@ -126,6 +126,8 @@ struct MixedScope {
   ~MixedScope() { g_mixed = prev; }
 };
 
+int device_cus();  // CUs of the current device (below)
+
 // Output tile size for a launch list.  64 by default: measured on the
 // selected inversion at n=256, b=1024 the 128-tile kernel (one workgroup
 // per CU, split-K 4) took 107.6 ms against 62.1 ms for the 64-tile kernel
@@ -165,17 +167,20 @@ int run_gemms(GemmList& list, hipStream_t s) {
     return ROCBTA_OK;
   }
   const int ts = gemm_ts(list);
-  static const bool h2 = [] {
+  // 512-thread 64-tile kernel for launches of at most one tile per CU (two
+  // waves per SIMD instead of one); ROCBTA_GEMM_H2=0 / 1 forces it off / on.
+  static const int h2_mode = [] {
     const char* e = getenv("ROCBTA_GEMM_H2");
-    return e && strcmp(e, "1") == 0;
+    return e ? (strcmp(e, "1") == 0 ? 1 : 0) : -1;
   }();
+  const int cus = device_cus();
   GemmGroup g{};
   auto flush = [&]() -> int {
     if (g.nprob == 0) return ROCBTA_OK;
     if (g.total_tiles > 0) {
       if (ts == 128)
         gemm_group_kernel_t128<<<g.total_tiles, GT_THREADS, 0, s>>>(g);
-      else if (h2)
+      else if (h2_mode == 1 || (h2_mode < 0 && g.total_tiles <= cus))
         gemm_group_kernel_h2<<<g.total_tiles, 2 * GT_THREADS, 0, s>>>(g);
       else
         gemm_group_kernel<<<g.total_tiles, GT_THREADS, 0, s>>>(g);

@@ -191,14 +191,54 @@ __device__ __forceinline__ void lds_syrk_update(double (*S)[TT_LD], int p, int w
     }
 }
 
+// Full 64-tile form: S[c0:64, c0:c1] (lower part) -= P P^T restricted to
+// the 16-wide column blocks [c0, c1), P = S[:, p:p+16].  Block t of the
+// enumeration goes to wave wave0 + t % nw; other waves return at once.
+__device__ __forceinline__ void lds_syrk_cols(double (*S)[TT_LD], int p, int c0, int c1,
+                                              int wave0, int nw) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int li = lane & 15, lk = lane >> 4;
+  int t = 0;
+  for (int cb = c0 / 16; cb < c1 / 16; ++cb)
+    for (int rb = cb; rb < TT / 16; ++rb, ++t) {
+      if (wave != wave0 + t % nw) continue;
+      f64x4 acc = {0.0, 0.0, 0.0, 0.0};
+      const int ra = rb * 16 + li, rc = cb * 16 + li;
+#pragma unroll
+      for (int kk = 0; kk < PW; kk += 4) acc = mfma_f64(S[ra][p + kk + lk], S[rc][p + kk + lk], acc);
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int row = rb * 16 + acc_row_f64(lane, r), col = cb * 16 + acc_col(lane);
+        if (col <= row) S[row][col] -= acc[r];
+      }
+    }
+}
+
 // Blocked Cholesky of S (m x m, lower part valid) in place.  MFIX = 64
-// compiles the full-tile case with all sizes constant; MFIX = 0 handles any
-// m <= 64.
+// compiles the full-tile case with all sizes constant and a look-ahead: after
+// panel p only the next panel's columns are updated before the next panel
+// is factored (wave 0), while waves 1-3 apply panel p to the columns beyond
+// it.  MFIX = 0 handles any m <= 64.
 template <int MFIX>
 __device__ __forceinline__ void lds_potrf_t(double (*S)[TT_LD], int m_rt, int* info,
                                             int64_t row0) {
   const int m = MFIX ? MFIX : m_rt;
   const int wave = threadIdx.x >> 6;
+  if (MFIX == TT) {
+    if (wave == 0) wave_panel_potrf16(S, 0, TT, info, row0);
+    __syncthreads();
+#pragma unroll
+    for (int p = 0; p + PW < TT; p += PW) {
+      lds_syrk_cols(S, p, p + PW, p + 2 * PW, 0, 4);  // next panel's columns
+      __syncthreads();
+      if (wave == 0)
+        wave_panel_potrf16(S, p + PW, TT, info, row0);
+      else if (p + 2 * PW < TT)
+        lds_syrk_cols(S, p, p + 2 * PW, TT, 1, 3);  // the rest, beside the panel
+      __syncthreads();
+    }
+    return;
+  }
 #pragma unroll
   for (int p = 0; p < m; p += PW) {
     const int w = min(PW, m - p);
