@@ -326,7 +326,7 @@ __device__ __forceinline__ void sw_emit(const SweepP& P, int chunk, int g, int c
 }
 
 template <int RC>
-__device__ void sw_tip_step(const SweepP& P, int chunk, int t, int cols, SwLds<RC>& S) {
+__device__ __forceinline__ void sw_tip_step(const SweepP& P, int chunk, int t, int cols, SwLds<RC>& S) {
   const int o = threadIdx.x & 63, kq = threadIdx.x >> 6;
   const bool fwd = P.trans == 0;
   const int g = P.NT + t;
@@ -387,7 +387,7 @@ __device__ void sw_tip_step(const SweepP& P, int chunk, int t, int cols, SwLds<R
 }
 
 template <int RC>
-__device__ void sw_chain(const SweepP& P, int chunk, SwLds<RC>& S) {
+__device__ __forceinline__ void sw_chain(const SweepP& P, int chunk, SwLds<RC>& S) {
   const int o = threadIdx.x & 63, kq = threadIdx.x >> 6;
   const int cols = min(RC, P.r - chunk * RC);
   const bool fwd = P.trans == 0;
@@ -457,7 +457,7 @@ __device__ void sw_chain(const SweepP& P, int chunk, SwLds<RC>& S) {
 // publishes the solution tiles they need, with the next factor tile
 // prefetched into registers.
 template <int RC>
-__device__ void sw_helper(const SweepP& P, int kind, int chunk, int g, SwLds<RC>& S) {
+__device__ __forceinline__ void sw_helper(const SweepP& P, int kind, int chunk, int g, SwLds<RC>& S) {
   const int cols = min(RC, P.r - chunk * RC);
   (void)cols;
   const bool fwd = P.trans == 0;
