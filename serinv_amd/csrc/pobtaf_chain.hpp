@@ -554,13 +554,40 @@ __device__ void chain_run(const PobtafChain& P, double* S, double* Xs, double* W
     __syncthreads();
     const int m = dt.rows;
     lds_potrf(S2, m, P.info, (int64_t)(g / P.nt) * P.b + (int64_t)(g % P.nt) * CT);
-    for (int idx = tid; idx < CT * CT; idx += CH_THREADS) {
-      const int rr = idx >> 6, cc = idx & 63;
-      if (rr < m && cc < m) dt.p[(int64_t)rr * dt.ld + cc] = (cc <= rr) ? S2[rr][cc] : 0.0;
+    // L(g,g) to memory with its upper triangle zero.  Full tiles: each
+    // thread reads its 8 row pairs from LDS first, then issues 16-byte
+    // stores (rows rs + 8q, columns c, c + 1).
+    const int rs = tid >> 5, c2 = (tid & 31) * 2;
+    if (m == CT && (dt.ld & 1) == 0 && (reinterpret_cast<uintptr_t>(dt.p) & 15) == 0) {
+      cdbl2 v[8];
+#pragma unroll
+      for (int q = 0; q < 8; ++q) {
+        const int rr = rs + 8 * q;
+        v[q].x = (c2 <= rr) ? S2[rr][c2] : 0.0;
+        v[q].y = (c2 + 1 <= rr) ? S2[rr][c2 + 1] : 0.0;
+      }
+#pragma unroll
+      for (int q = 0; q < 8; ++q)
+        *reinterpret_cast<cdbl2*>(dt.p + (int64_t)(rs + 8 * q) * dt.ld + c2) = v[q];
+    } else {
+      for (int idx = tid; idx < CT * CT; idx += CH_THREADS) {
+        const int rr = idx >> 6, cc = idx & 63;
+        if (rr < m && cc < m) dt.p[(int64_t)rr * dt.ld + cc] = (cc <= rr) ? S2[rr][cc] : 0.0;
+      }
     }
     lds_trtri<false>(S2, X2, W2, m);
     double* li_out = P.linv + (int64_t)g * CT * CT;
-    for (int idx = tid; idx < CT * CT; idx += CH_THREADS) li_out[idx] = X2[idx >> 6][idx & 63];
+    {
+      cdbl2 v[8];
+#pragma unroll
+      for (int q = 0; q < 8; ++q) {
+        v[q].x = X2[rs + 8 * q][c2];
+        v[q].y = X2[rs + 8 * q][c2 + 1];
+      }
+#pragma unroll
+      for (int q = 0; q < 8; ++q)
+        *reinterpret_cast<cdbl2*>(li_out + (rs + 8 * q) * CT + c2) = v[q];
+    }
     // Linv(g) is published before the sub-diagonal solve: the helpers'
     // solves of column g (TRSM(g+2,g) feeds PX(g+1), which the next step
     // waits for) start while the chain finishes this step.
