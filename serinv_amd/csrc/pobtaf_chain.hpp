@@ -406,6 +406,31 @@ __device__ __forceinline__ void cacc_to_lds(const f64x4 (&acc)[2][2], double* X,
             acc[i][j][r];
 }
 
+// Store a tile staged in LDS (X[row * pitch + col]) to memory.  Full,
+// 16-byte aligned tiles: each thread reads its 8 row pairs (rows rs + 8q,
+// columns c, c + 1) first and then issues 16-byte stores; other tiles go
+// element by element.
+__device__ __forceinline__ void lds_tile_store(const double* X, int pitch, const CTile& t) {
+  const int tid = threadIdx.x;
+  if (t.rows == CT && t.cols == CT && (t.ld & 1) == 0 &&
+      (reinterpret_cast<uintptr_t>(t.p) & 15) == 0) {
+    const int rs = tid >> 5, c = (tid & 31) * 2;
+    cdbl2 v[8];
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+      v[q].x = X[(rs + 8 * q) * pitch + c];
+      v[q].y = X[(rs + 8 * q) * pitch + c + 1];
+    }
+#pragma unroll
+    for (int q = 0; q < 8; ++q) *reinterpret_cast<cdbl2*>(t.p + (int64_t)(rs + 8 * q) * t.ld + c) = v[q];
+  } else {
+    for (int idx = tid; idx < CT * CT; idx += CH_THREADS) {
+      const int row = idx >> 6, col = idx & 63;
+      if (row < t.rows && col < t.cols) t.p[(int64_t)row * t.ld + col] = X[row * pitch + col];
+    }
+  }
+}
+
 // Zero a whole tile (used for the strict-upper tiles of the D blocks).
 __device__ __forceinline__ void tile_zero(const CTile& t) {
   for (int idx = threadIdx.x; idx < CT * CT; idx += CH_THREADS) {
@@ -426,22 +451,42 @@ __device__ __forceinline__ void helper_products(const PobtafChain& P, const Cont
                                                 double* XB, ACC (&acc)[2][2]) {
   TileRegs16 ra, rb;
   auto tileA = [&](int k) { return cr.ra < 0 ? tile_arrow(P, k) : tile_g(P, cr.ra, k); };
+  __shared__ int s_ready;
   chain_wait(P, factor_flag(P, cr.ra, cr.k0), factor_flag(P, cr.jb, cr.k0));
   tile_load16(tileA(cr.k0), ra);
   tile_load16(tile_g(P, cr.jb, cr.k0), rb);
   for (int k = cr.k0;; ++k) {
+    const bool more = k + 1 < cr.k1;
     __syncthreads();  // previous product done with XA / XB
     tile_store16(ra, XA);
     tile_store16(rb, XB);
+    // The next contribution is prefetched only if its tiles are already
+    // published (checked without waiting); otherwise this product runs
+    // first and the wait follows it, so that the last product of a task is
+    // not held behind the wait for it.
+    if (threadIdx.x == 0) {
+      int ready = 0;
+      if (more &&
+          __hip_atomic_load(factor_flag(P, cr.ra, k + 1), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) &&
+          __hip_atomic_load(factor_flag(P, cr.jb, k + 1), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+        ready = 1;
+      }
+      s_ready = ready;
+    }
     __syncthreads();
-    const bool more = k + 1 < cr.k1;
-    if (more) {
-      chain_wait(P, factor_flag(P, cr.ra, k + 1), factor_flag(P, cr.jb, k + 1));
+    const bool pre = more && s_ready;
+    if (pre) {
       tile_load16(tileA(k + 1), ra);
       tile_load16(tile_g(P, cr.jb, k + 1), rb);
     }
     mfma_abt(XA, XB, acc, sgn, lower_only);
     if (!more) break;
+    if (!pre) {
+      chain_wait(P, factor_flag(P, cr.ra, k + 1), factor_flag(P, cr.jb, k + 1));
+      tile_load16(tileA(k + 1), ra);
+      tile_load16(tile_g(P, cr.jb, k + 1), rb);
+    }
   }
   __syncthreads();
 }
@@ -642,9 +687,10 @@ __device__ void chain_run(const PobtafChain& P, double* S, double* Xs, double* W
         acc[1][1] = mfma_f64(a1, b1, acc[1][1]);
       }
     }
-    cacc_store(xt, acc);
     __syncthreads();  // S no longer read as X
     cacc_to_lds(acc, S, TT_LD);  // keep L(g+1,g) as S[row][k]
+    __syncthreads();
+    lds_tile_store(S, TT_LD, xt);
     chain_publish(P.cflags + 2 * g + 1);
   }
 }
@@ -723,7 +769,10 @@ __device__ __forceinline__ void chain_helper_task(const PobtafChain& P, int task
   const ContribRange cr = chain_contribs(P, kind, r, j);
   helper_accumulate<TAG>(P, cr, -1.0, kind == HK_PD, XA, XB, acc);
   if (kind == HK_TRSM || kind == HK_ARR) helper_solve<TAG>(P, j, XA, XB, acc);
-  cacc_store(out, acc);
+  __syncthreads();  // the last product is done with XA
+  cacc_to_lds(acc, XA, KP);
+  __syncthreads();
+  lds_tile_store(XA, KP, out);
   chain_publish(P.hflags + task);
   // The strict-upper mirror of a lower D-block tile is zero in the factor.
   // No task reads it, so it is cleared after the publish, off the path of
