@@ -246,6 +246,17 @@ __device__ __forceinline__ void chain_wait(const PobtafChain& P, const int* fa, 
   __syncthreads();
 }
 
+// The calling wave alone waits for fa (and fb if non-null), then acquires.
+// All lanes poll the same address, so the loop is uniform; no barrier.
+__device__ __forceinline__ void wave_wait(const PobtafChain& P, const int* fa, const int* fb) {
+  long long spins = 0;
+  bool ok = chain_poll(P, fa, spins);
+  if (ok && fb) ok = chain_poll(P, fb, spins);
+  if (!ok && (threadIdx.x & 63) == 0)
+    __hip_atomic_store(P.info, -1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+}
+
 // Every wave waits for its own global stores to complete (s_waitcnt
 // vmcnt(0)) before the barrier: a workgroup barrier does not wait for them,
 // and the release by thread 0 (L2 write-back, then its own wave's counter)
@@ -492,7 +503,7 @@ __device__ __forceinline__ void helper_products(const PobtafChain& P, const Cont
 }
 
 template <int TAG>
-__device__ void helper_accumulate(const PobtafChain& P, const ContribRange& cr, double sgn,
+__device__ __forceinline__ void helper_accumulate(const PobtafChain& P, const ContribRange& cr, double sgn,
                                   bool lower_only, double* XA, double* XB, f64x4 (&acc)[2][2]) {
   if (cr.k0 >= cr.k1) return;
   if (P.f32) {
@@ -507,7 +518,7 @@ __device__ void helper_accumulate(const PobtafChain& P, const ContribRange& cr, 
 
 // out = acc * Linv(j)^T
 template <int TAG>
-__device__ void helper_solve(const PobtafChain& P, int j, double* XA, double* XB,
+__device__ __forceinline__ void helper_solve(const PobtafChain& P, int j, double* XA, double* XB,
                              f64x4 (&acc)[2][2]) {
   chain_wait(P, P.cflags + 2 * j, nullptr);
   TileRegs16 rl;
@@ -531,7 +542,7 @@ __device__ void helper_solve(const PobtafChain& P, int j, double* XA, double* XB
 // K-contiguous with pitch 65 for the next step's update), Xs [64][65]
 // (Linv(g)), Ws [64][17] (trtri scratch).
 template <int TAG>
-__device__ void chain_run(const PobtafChain& P, double* S, double* Xs, double* Ws) {
+__device__ __forceinline__ void chain_run(const PobtafChain& P, double* S, double* Xs, double* Ws) {
   const int tid = threadIdx.x;
   double (*S2)[TT_LD] = reinterpret_cast<double (*)[TT_LD]>(S);
   double (*X2)[TT_LD] = reinterpret_cast<double (*)[TT_LD]>(Xs);
@@ -620,6 +631,17 @@ __device__ void chain_run(const PobtafChain& P, double* S, double* Xs, double* W
         if (rr < m && cc < m) dt.p[(int64_t)rr * dt.ld + cc] = (cc <= rr) ? S2[rr][cc] : 0.0;
       }
     }
+    // PX(g) and PD(g+1) are normally published before the inversion starts.
+    // Each wave reads both flags now and checks them after the Linv publish:
+    // when both were set it only acquires, without the poll's round trip.
+    const bool sub = g + 1 < band_end(P, g);
+    const bool pre = sub && g + 1 < P.NT;
+    const int* f_px = P.hflags + t_px(P, g);
+    const int* f_pd = pre ? P.hflags + t_pd(P, g + 1) : f_px;
+    int seen = 0;
+    if (sub)
+      seen = __hip_atomic_load(f_px, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) &
+             __hip_atomic_load(f_pd, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     lds_trtri<false>(S2, X2, W2, m);
     double* li_out = P.linv + (int64_t)g * CT * CT;
     {
@@ -637,10 +659,14 @@ __device__ void chain_run(const PobtafChain& P, double* S, double* Xs, double* W
     // solves of column g (TRSM(g+2,g) feeds PX(g+1), which the next step
     // waits for) start while the chain finishes this step.
     chain_publish(P.cflags + 2 * g);
-    if (g + 1 >= band_end(P, g)) continue;
-    // L(g+1,g) = PX(g) * Linv(g)^T
-    const bool pre = g + 1 < P.NT;
-    chain_wait(P, P.hflags + t_px(P, g), pre ? P.hflags + t_pd(P, g + 1) : nullptr);
+    if (!sub) continue;
+    // L(g+1,g) = PX(g) * Linv(g)^T.  Each thread loads only its own parts of
+    // the two tiles, so the waves acquire independently; the barrier after
+    // the staging below orders the LDS.
+    if (__builtin_amdgcn_readfirstlane(seen))
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+    else
+      wave_wait(P, f_px, pre ? f_pd : nullptr);
     const CTile xt = tile_g(P, g + 1, g);
     {
       TileRegs16 rx;
