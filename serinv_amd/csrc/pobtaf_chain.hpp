@@ -559,6 +559,7 @@ __device__ __forceinline__ void chain_run(const PobtafChain& P, double* S, doubl
   // that the next step starts without the load latency.
   f64x4 pd_next[2][2];
   bool have_pd = false;
+  __shared__ int s_early;
   for (int g = 0; g < P.NT; ++g) {
     const CTile dt = tile_g(P, g, g);
     f64x4 acc[2][2];
@@ -609,7 +610,34 @@ __device__ __forceinline__ void chain_run(const PobtafChain& P, double* S, doubl
     cacc_to_lds(acc, S, TT_LD);
     __syncthreads();
     const int m = dt.rows;
-    lds_potrf(S2, m, P.info, (int64_t)(g / P.nt) * P.b + (int64_t)(g % P.nt) * CT);
+    const bool sub = g + 1 < band_end(P, g);
+    const bool pre = sub && g + 1 < P.NT;
+    const int* f_px = P.hflags + t_px(P, g);
+    const int* f_pd = pre ? P.hflags + t_pd(P, g + 1) : f_px;
+    // PX(g) and PD(g+1) are normally published while the tile is factored.
+    // Wave 1 checks their flags while wave 0 factors the last panel and, if
+    // both are set, acquires; every thread then loads its parts of the two
+    // tiles after its L(g,g) stores (which would otherwise queue behind the
+    // loads), and the loads land during trtri.
+    {
+      const int64_t row0 = (int64_t)(g / P.nt) * P.b + (int64_t)(g % P.nt) * CT;
+      if (m == CT) {
+        lds_potrf_t<CT>(S2, CT, P.info, row0, [&](int p) {
+          if (p == 32 && wave == 1) {
+            int ok = sub ? (__hip_atomic_load(f_px, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) &
+                            __hip_atomic_load(f_pd, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
+                         : 0;
+            ok = __builtin_amdgcn_readfirstlane(ok);
+            if (ok) __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+            if (lane == 0) s_early = ok;
+          }
+        });
+      } else {
+        if (tid == 0) s_early = 0;
+        lds_potrf_t<0>(S2, m, P.info, row0);
+      }
+    }
+    const int early = s_early;
     // L(g,g) to memory with its upper triangle zero.  Full tiles: each
     // thread reads its 8 row pairs from LDS first, then issues 16-byte
     // stores (rows rs + 8q, columns c, c + 1).
@@ -631,15 +659,17 @@ __device__ __forceinline__ void chain_run(const PobtafChain& P, double* S, doubl
         if (rr < m && cc < m) dt.p[(int64_t)rr * dt.ld + cc] = (cc <= rr) ? S2[rr][cc] : 0.0;
       }
     }
-    // PX(g) and PD(g+1) are normally published before the inversion starts.
-    // Each wave reads both flags now and checks them after the Linv publish:
-    // when both were set it only acquires, without the poll's round trip.
-    const bool sub = g + 1 < band_end(P, g);
-    const bool pre = sub && g + 1 < P.NT;
-    const int* f_px = P.hflags + t_px(P, g);
-    const int* f_pd = pre ? P.hflags + t_pd(P, g + 1) : f_px;
+    const CTile xt = tile_g(P, g + 1, g);
+    TileRegs16 rx;
+    if (early) {
+      tile_load16(xt, rx);
+      if (pre) cacc_load(tile_g(P, g + 1, g + 1), pd_next);
+    }
+    // Otherwise each wave reads both flags again now and checks them after
+    // the Linv publish: when both were set it only acquires, without the
+    // poll's round trip.
     int seen = 0;
-    if (sub)
+    if (sub && !early)
       seen = __hip_atomic_load(f_px, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) &
              __hip_atomic_load(f_pd, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     lds_trtri<false>(S2, X2, W2, m);
@@ -663,18 +693,16 @@ __device__ __forceinline__ void chain_run(const PobtafChain& P, double* S, doubl
     // L(g+1,g) = PX(g) * Linv(g)^T.  Each thread loads only its own parts of
     // the two tiles, so the waves acquire independently; the barrier after
     // the staging below orders the LDS.
-    if (__builtin_amdgcn_readfirstlane(seen))
-      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-    else
-      wave_wait(P, f_px, pre ? f_pd : nullptr);
-    const CTile xt = tile_g(P, g + 1, g);
-    {
-      TileRegs16 rx;
+    if (!early) {
+      if (__builtin_amdgcn_readfirstlane(seen))
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+      else
+        wave_wait(P, f_px, pre ? f_pd : nullptr);
       tile_load16(xt, rx);
-      if (pre) {
-        cacc_load(tile_g(P, g + 1, g + 1), pd_next);
-        have_pd = true;
-      }
+      if (pre) cacc_load(tile_g(P, g + 1, g + 1), pd_next);
+    }
+    have_pd = pre;
+    {
       // S[m][k] = X(m, k) (pitch 65 rows are 8-byte aligned only: scalar stores)
       const int rs = tid >> 5, c = (tid & 31) * 2;
 #pragma unroll

@@ -214,14 +214,20 @@ __device__ __forceinline__ void lds_syrk_cols(double (*S)[TT_LD], int p, int c0,
     }
 }
 
+struct NoIdle {
+  __device__ void operator()(int) const {}
+};
+
 // Blocked Cholesky of S (m x m, lower part valid) in place.  MFIX = 64
 // compiles the full-tile case with all sizes constant and a look-ahead: after
 // panel p only the next panel's columns are updated before the next panel
 // is factored (wave 0), while waves 1-3 apply panel p to the columns beyond
-// it.  MFIX = 0 handles any m <= 64.
-template <int MFIX>
+// it.  Full tiles: after that update (if any), waves 1-3 run idle(p) while
+// wave 0 factors panel p + 16; panels 0 .. p are final by then.  The call
+// ends with a barrier after the last of them.  MFIX = 0 handles any m <= 64.
+template <int MFIX, typename IDLE = NoIdle>
 __device__ __forceinline__ void lds_potrf_t(double (*S)[TT_LD], int m_rt, int* info,
-                                            int64_t row0) {
+                                            int64_t row0, IDLE idle = IDLE()) {
   const int m = MFIX ? MFIX : m_rt;
   const int wave = threadIdx.x >> 6;
   if (MFIX == TT) {
@@ -233,8 +239,10 @@ __device__ __forceinline__ void lds_potrf_t(double (*S)[TT_LD], int m_rt, int* i
       __syncthreads();
       if (wave == 0)
         wave_panel_potrf16(S, p + PW, TT, info, row0);
-      else if (p + 2 * PW < TT)
-        lds_syrk_cols(S, p, p + 2 * PW, TT, 1, 3);  // the rest, beside the panel
+      else {
+        if (p + 2 * PW < TT) lds_syrk_cols(S, p, p + 2 * PW, TT, 1, 3);  // the rest, beside the panel
+        idle(p);
+      }
       __syncthreads();
     }
     return;
