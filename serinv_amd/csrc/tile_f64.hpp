@@ -146,14 +146,21 @@ __device__ __forceinline__ void wave_panel_potrf16(double (*S)[TT_LD], int p, in
       if (act && p + c0 + q <= i) S[i][p + c0 + q] = r[c0 + q];
     if (c0 + 4 < PW) {
       __builtin_amdgcn_wave_barrier();  // the sub-panel's LDS stores precede the reads below
+      // Every broadcast read of the finished rows is issued before the next
+      // pivot column's update, which takes readlanes: the reads' latency
+      // then overlaps it instead of stalling the pivot path read by read.
+      double lrow[PW - 5][4];
+#pragma unroll
+      for (int k = c0 + 5; k < PW; ++k)
+#pragma unroll
+        for (int q = 0; q < 4; ++q) lrow[k - c0 - 5][q] = S[p + k][p + c0 + q];
+      __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
       for (int q = 0; q < 4; ++q) r[c0 + 4] -= r[c0 + q] * readlane_f64(r[c0 + q], c0 + 4);
 #pragma unroll
-      for (int k = c0 + 5; k < PW; ++k) {
-        const double* row = &S[p + k][p + c0];
+      for (int k = c0 + 5; k < PW; ++k)
 #pragma unroll
-        for (int q = 0; q < 4; ++q) r[k] -= r[c0 + q] * row[q];
-      }
+        for (int q = 0; q < 4; ++q) r[k] -= r[c0 + q] * lrow[k - c0 - 5][q];
       __builtin_amdgcn_wave_barrier();
     }
   }
