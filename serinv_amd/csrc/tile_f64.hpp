@@ -300,13 +300,32 @@ __device__ __forceinline__ void wave_diag_inv16(const double (*S)[TT_LD], double
   const int lane = threadIdx.x & 63;
   const int c = lane & 15;
   const double dinv = 1.0 / S[o + c][o + c];
+  const double* S0 = &S[0][0];
   double x[16];
+  // The LDS broadcast reads of row i+1, L(i+1, 0..i), are issued while row i
+  // is substituted, so each row's reads have landed when it starts.  Left to
+  // itself the compiler either waits after every read or hoists every row's
+  // reads to the top (all 120 entries live at once).  The read address of
+  // row i+1 takes a zero produced from x[i-1] (an asm move with that input),
+  // which keeps the reads below x[i-1]; the scheduling barrier keeps them
+  // above row i's work.
+  double cur[16], nxt[16];
 #pragma unroll
   for (int i = 0; i < 16; ++i) {
+    if (i + 1 < 16) {
+      int zero = 0;
+      if (i >= 1) asm volatile("v_mov_b32 %0, 0" : "=v"(zero) : "v"(x[i - 1]));
+      const double* row = S0 + (o + i + 1) * TT_LD + o + zero;
+#pragma unroll
+      for (int k = 0; k <= i; ++k) nxt[k] = row[k];
+    }
+    __builtin_amdgcn_sched_barrier(0);
     double acc = (i == c) ? 1.0 : 0.0;
 #pragma unroll
-    for (int k = 0; k < i; ++k) acc -= S[o + i][o + k] * x[k];
+    for (int k = 0; k < i; ++k) acc -= cur[k] * x[k];
     x[i] = acc * readlane_f64(dinv, i);
+#pragma unroll
+    for (int k = 0; k <= i; ++k) cur[k] = nxt[k];
   }
   if (lane < 16) {
 #pragma unroll
