@@ -614,22 +614,44 @@ __device__ __forceinline__ void chain_run(const PobtafChain& P, double* S, doubl
     const bool pre = sub && g + 1 < P.NT;
     const int* f_px = P.hflags + t_px(P, g);
     const int* f_pd = pre ? P.hflags + t_pd(P, g + 1) : f_px;
+    // Full tiles: the inverse Linv(g) is formed by 16-row block rows while
+    // the tile is factored.  Block row I needs only panels 0 .. I, so waves
+    // 1-3 form block rows 0, 1, 2 while wave 0 factors panels 1, 2, 3
+    // (X_II by forward substitution, X_IJ = -X_II sum_K L_IK X_KJ on MFMA);
+    // block row 3 follows the factorization.  A wave that needs X_22 or X_33
+    // before another wave's copy is visible forms its own copy in Ws.
     // PX(g) and PD(g+1) are normally published while the tile is factored.
     // Wave 1 checks their flags while wave 0 factors the last panel and, if
     // both are set, acquires; every thread then loads its parts of the two
     // tiles after its L(g,g) stores (which would otherwise queue behind the
-    // loads), and the loads land during trtri.
+    // loads), and the loads land during the rest of the inversion.
     {
       const int64_t row0 = (int64_t)(g / P.nt) * P.b + (int64_t)(g % P.nt) * CT;
       if (m == CT) {
         lds_potrf_t<CT>(S2, CT, P.info, row0, [&](int p) {
-          if (p == 32 && wave == 1) {
-            int ok = sub ? (__hip_atomic_load(f_px, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) &
-                            __hip_atomic_load(f_pd, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
-                         : 0;
-            ok = __builtin_amdgcn_readfirstlane(ok);
-            if (ok) __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-            if (lane == 0) s_early = ok;
+          if (p == 0 && wave == 3) wave_diag_inv16(S2, X2, 0);
+          if (p == 16 && wave == 2) {
+            wave_diag_inv16(S2, X2, 16);
+            __builtin_amdgcn_wave_barrier();
+            wave_trtri_offdiag(S2, X2, &X2[16][16], TT_LD, 1, 0);
+          }
+          if (p == 32) {
+            if (wave == 1) {
+              int ok = sub ? (__hip_atomic_load(f_px, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) &
+                              __hip_atomic_load(f_pd, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
+                           : 0;
+              ok = __builtin_amdgcn_readfirstlane(ok);
+              if (ok) __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+              if (lane == 0) s_early = ok;
+            } else if (wave == 2) {
+              wave_diag_inv16(S2, X2, 32);
+              __builtin_amdgcn_wave_barrier();
+              wave_trtri_offdiag(S2, X2, &X2[32][32], TT_LD, 2, 1);
+            } else {
+              wave_diag_inv16_to(S2, 32, Ws, WS_LD);
+              __builtin_amdgcn_wave_barrier();
+              wave_trtri_offdiag(S2, X2, Ws, WS_LD, 2, 0);
+            }
           }
         });
       } else {
@@ -672,7 +694,20 @@ __device__ __forceinline__ void chain_run(const PobtafChain& P, double* S, doubl
     if (sub && !early)
       seen = __hip_atomic_load(f_px, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) &
              __hip_atomic_load(f_pd, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    lds_trtri<false>(S2, X2, W2, m);
+    if (m == CT) {
+      // block row 3: X_33 on waves 0-2 (wave 0's copy is the result), then
+      // X_32, X_31, X_30
+      if (wave < 3) {
+        double* xi = (wave == 0) ? &X2[48][48] : Ws + 16 * wave * WS_LD;
+        const int ldi = (wave == 0) ? TT_LD : WS_LD;
+        wave_diag_inv16_to(S2, 48, xi, ldi);
+        __builtin_amdgcn_wave_barrier();
+        wave_trtri_offdiag(S2, X2, xi, ldi, 3, 2 - wave);
+      }
+      __syncthreads();
+    } else {
+      lds_trtri<false>(S2, X2, W2, m);
+    }
     double* li_out = P.linv + (int64_t)g * CT * CT;
     {
       cdbl2 v[8];

@@ -295,8 +295,9 @@ constexpr int WS_LD = 17;
 // substitution, x_i = (delta_ic - sum_{k<i} L_ik x_k) / L_ii; the entries
 // L_ik are LDS broadcasts and 1 / L_ii comes from lane i by readlane, so the
 // step needs no division on its dependency path and no cross-lane shuffle.
-__device__ __forceinline__ void wave_diag_inv16(const double (*S)[TT_LD], double (*Xs)[TT_LD],
-                                                int o) {
+// The block is written to dst (row pitch ld).
+__device__ __forceinline__ void wave_diag_inv16_to(const double (*S)[TT_LD], int o, double* dst,
+                                                   int ld) {
   const int lane = threadIdx.x & 63;
   const int c = lane & 15;
   const double dinv = 1.0 / S[o + c][o + c];
@@ -329,8 +330,35 @@ __device__ __forceinline__ void wave_diag_inv16(const double (*S)[TT_LD], double
   }
   if (lane < 16) {
 #pragma unroll
-    for (int i = 0; i < 16; ++i) Xs[o + i][o + c] = x[i];
+    for (int i = 0; i < 16; ++i) dst[i * ld + c] = x[i];
   }
+}
+__device__ __forceinline__ void wave_diag_inv16(const double (*S)[TT_LD], double (*Xs)[TT_LD],
+                                                int o) {
+  wave_diag_inv16_to(S, o, &Xs[o][o], TT_LD);
+}
+
+// Off-diagonal 16x16 block of X = L^{-1} by the calling wave (J < I):
+//   X_IJ = -X_II * W,  W = sum_{K=J}^{I-1} L_IK X_KJ,
+// with L in S, the blocks X_KJ (K < I) in Xs, and X_II read from XI (row
+// pitch ldi).  X_IJ is written to Xs.  Lane (q, j) = (lane >> 4, lane & 15)
+// holds W(4r + q, j) in w[r] after the first product, which is the B operand
+// of k-step r of the second, so W never leaves the registers.
+__device__ __forceinline__ void wave_trtri_offdiag(const double (*S)[TT_LD], double (*Xs)[TT_LD],
+                                                   const double* XI, int ldi, int I, int J) {
+  const int lane = threadIdx.x & 63, li = lane & 15, lk = lane >> 4;
+  f64x4 w = {0.0, 0.0, 0.0, 0.0};
+  for (int K = J; K < I; ++K)
+#pragma unroll
+    for (int kk = 0; kk < 16; kk += 4) {
+      const int k = K * 16 + kk + lk;
+      w = mfma_f64(S[I * 16 + li][k], Xs[k][J * 16 + li], w);
+    }
+  f64x4 x = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+  for (int r = 0; r < 4; ++r) x = mfma_f64(XI[li * ldi + 4 * r + lk], w[r], x);
+#pragma unroll
+  for (int r = 0; r < 4; ++r) Xs[I * 16 + acc_row_f64(lane, r)][J * 16 + acc_col(lane)] = -x[r];
 }
 
 // ZERO = false: the caller guarantees that the blocks of Xs strictly above
