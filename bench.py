@@ -7,12 +7,12 @@ timed steps), in ms.  ms_per_step is the wall time per step of the bracketed
 region (restore copy included).  pobtas (1 and 64 RHS) is timed separately.
 
 The CPU baseline is the NumPy restatement of the reference algorithm
-(oracle/serinv_np.py) timed on this host's cores on a sample of the same
-workload (the first n_s block rows, b=1024, a=32), scaled to n=256 by the
-algorithmic FLOP ratio.
+(oracle/serinv_np.py) run once on the whole workload on this host's cores
+(CPU-2, not scaled); it is also the parity oracle of the GPU outputs at this
+size (`parity_vs_oracle`).  CPU-1 (n=32, b=64, a=16) is timed beside it.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--n N --b B --a A]
-                       [--no-cpu-baseline] [--profile-steps K]
+                       [--no-cpu-baseline] [--no-solve] [--no-mixed] [--no-batched]
 """
 
 import argparse
@@ -40,7 +40,6 @@ def parse():
     p.add_argument("--gamma", type=float, default=None,
                    help="diagonal-shift scale (default: calibrated value for kappa ~ 1e6)")
     p.add_argument("--no-cpu-baseline", action="store_true")
-    p.add_argument("--cpu-seconds", type=float, default=15.0)
     p.add_argument("--no-solve", action="store_true")
     p.add_argument("--no-batched", action="store_true")
     p.add_argument("--no-mixed", action="store_true")
@@ -71,50 +70,93 @@ def measured_traffic(n, b, a):
     return None
 
 
-def cpu_baseline(b, a, n_full, budget_s):
-    """Time the NumPy restatement on the first n_s block rows; scale by FLOPs."""
+def _cpu_info():
+    import numpy as np
+    from threadpoolctl import threadpool_info
+
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    info = threadpool_info()
+    blas = ",".join(sorted({f"{i.get('internal_api')} {i.get('version')}" for i in info}))
+    return model, blas, np.__version__
+
+
+def cpu1_baseline(reps=5):
+    """CPU-1: the oracle on n=32, b=64, a=16, r=1 (strictly diagonally
+    dominant shifts), one warm-up, then the median of `reps` runs on fresh
+    copies; per-op milliseconds."""
+    import numpy as np
+
+    from oracle import serinv_np
+    from serinv_amd.utils.matrices import bta_spd
+
+    base = bta_spd(32, 64, 16, seed=63, n_rhs=1)
+    times = []
+    for k in range(reps + 1):
+        D, L, R, T, B = (x.copy() for x in base)
+        t0 = time.perf_counter()
+        serinv_np.pobtaf(D, L, R, T)
+        t1 = time.perf_counter()
+        serinv_np.pobtas(D, L, R, T, B, trans="N")
+        serinv_np.pobtas(D, L, R, T, B, trans="C")
+        t2 = time.perf_counter()
+        serinv_np.pobtasi(D, L, R, T)
+        t3 = time.perf_counter()
+        if k:
+            times.append((t1 - t0, t3 - t2, t2 - t1))
+    med = np.median(np.array(times), axis=0) * 1e3
+    return {"workload": "n=32 b=64 a=16 r=1", "pobtaf_ms": round(float(med[0]), 3),
+            "pobtasi_ms": round(float(med[1]), 3), "pobtas_NC_ms": round(float(med[2]), 3),
+            "runs": reps}
+
+
+def cpu2_oracle(host, gpu, n, b, a):
+    """CPU-2: the oracle (oracle/serinv_np.py, the NumPy restatement of the
+    reference routines) on the full workload, once, on all host cores.  It
+    is the timed CPU baseline and the parity oracle of the GPU outputs at
+    this size: every output of the GPU run in `gpu` is compared with it.
+    Error metric per output array Y: max |Y - Y_ref| / max |Y_ref| over the
+    stored blocks (relative |difference| for the log-determinant)."""
     import numpy as np
     from threadpoolctl import threadpool_info, threadpool_limits
 
     from oracle import serinv_np
-    from serinv_amd.utils.matrices import bta_flops, bta_spd
+
+    def rel(x, ref):
+        return float(f"{np.max(np.abs(x - ref)) / np.max(np.abs(ref)):.3e}") if ref.size else 0.0
 
     cores = len(os.sched_getaffinity(0))
+    D, L, R, T, Bm = host
+    errs = {}
     with threadpool_limits(limits=cores):
-        info = threadpool_info()
-        threads = max([i.get("num_threads", 1) for i in info] or [1])
-
-        def run(ns):
-            D, L, R, T, _ = bta_spd(ns, b, a, seed=63)
+        threads = max([i.get("num_threads", 1) for i in threadpool_info()] or [1])
+        t0 = time.perf_counter()
+        serinv_np.pobtaf(D, L, R, T)
+        t_af = time.perf_counter() - t0
+        errs["L"] = {nm: rel(g, o) for nm, g, o in zip("DLRT", gpu.pop("L"), (D, L, R, T))}
+        ld = serinv_np.logdet_from_factor(D, T)
+        errs["logdet"] = float(f"{abs(gpu['logdet'] - ld) / abs(ld):.3e}")
+        t_sv = {}
+        for r in (1, 64):
+            x = np.ascontiguousarray(Bm[:, :r])
             t0 = time.perf_counter()
-            serinv_np.pobtaf(D, L, R, T)
-            t1 = time.perf_counter()
-            serinv_np.pobtasi(D, L, R, T)
-            t2 = time.perf_counter()
-            return t1 - t0, t2 - t1
-
-        run(2)  # warm-up
-        ns = 4
-        taf, tsi = run(ns)
-        per = (taf + tsi) / max(sum(bta_flops(ns, b, a)), 1.0)
-        ns2 = int(min(64, max(4, budget_s / max(per * sum(bta_flops(2, b, a)) / 2, 1e-9))))
-        if ns2 > ns:
-            ns = ns2
-            taf, tsi = run(ns)
-    f_s = sum(bta_flops(ns, b, a))
-    f_full = sum(bta_flops(n_full, b, a))
-    ms_full = (taf + tsi) * f_full / f_s * 1e3
-    blas = ",".join(sorted({f"{i.get('internal_api')} {i.get('version')}" for i in info}))
-    return {
-        "value": round(ms_full, 1),
-        "unit": "ms (pobtaf+pobtasi, scaled to the full workload)",
-        "cores": threads,
-        "kind": "port",
-        "sample": f"oracle/serinv_np.py pobtaf+pobtasi on n={ns} b={b} a={a} "
-                  f"({taf + tsi:.2f} s measured: pobtaf {taf:.2f} s, pobtasi {tsi:.2f} s), "
-                  f"scaled by FLOP ratio {f_full / f_s:.2f}; BLAS {blas}",
-        "tflops": round(f_s / (taf + tsi) * 1e-12, 4),
-    }
+            serinv_np.pobtas(D, L, R, T, x, trans="N")
+            serinv_np.pobtas(D, L, R, T, x, trans="C")
+            t_sv[r] = time.perf_counter() - t0
+            if f"x_r{r}" in gpu:
+                errs[f"x_r{r}"] = rel(gpu.pop(f"x_r{r}"), x)
+        t0 = time.perf_counter()
+        serinv_np.pobtasi(D, L, R, T)
+        t_si = time.perf_counter() - t0
+        errs["X"] = {nm: rel(g, o) for nm, g, o in zip("DLRT", gpu.pop("X"), (D, L, R, T))}
+    return t_af, t_si, t_sv, threads, errs
 
 
 def bench_batched(dev, nmat=64, r=4, reps=3, tune=True):
@@ -163,6 +205,7 @@ def bench_batched(dev, nmat=64, r=4, reps=3, tune=True):
         e1.record(stream)
         torch.cuda.synchronize(dev)
         loop.append(e0.elapsed_time(e1))
+    algs.check_status()
     loop_out = [[t.clone() for t in w] for w in work]
     tuned = None
     if tune:
@@ -294,6 +337,7 @@ def bench_mixed(dev, n, b, a, kappas=(1e4, 1e6, 1e8), r=16, reps=2):
         })
         del A, F64, X64, F32, X32
         torch.cuda.empty_cache()
+    algs.check_status()
     algs.config["check_info"] = True
     return out
 
@@ -322,6 +366,7 @@ def main():
           file=sys.stderr, flush=True)
     D0, L0, R0, T0 = (torch.from_numpy(x).to(dev) for x in (D, L, R, T))
     B0 = torch.from_numpy(Bm).to(dev)
+    host = (D, L, R, T, Bm)  # the oracle works on these in place, after the GPU runs
     del D, L, R, T
     Dw, Lw, Rw, Tw = (torch.empty_like(x) for x in (D0, L0, R0, T0))
     stream = torch.cuda.current_stream(dev)
@@ -353,6 +398,7 @@ def main():
         step(k)
     torch.cuda.synchronize()
     wall = time.perf_counter() - t_start
+    algs.check_status()  # no pivot failure or scheduling time-out in the timed calls
     t_af = [ev[k][0].elapsed_time(ev[k][1]) for k in range(args.steps)]
     t_si = [ev[k][1].elapsed_time(ev[k][2]) for k in range(args.steps)]
     t_tot = [x + y for x, y in zip(t_af, t_si)]
@@ -362,14 +408,22 @@ def main():
     print(f"[bench] pobtaf {np.mean(t_af):.2f} ms  pobtasi {np.mean(t_si):.2f} ms  "
           f"total {ms:.2f} ms  {tflops:.2f} TFLOP/s", file=sys.stderr, flush=True)
 
-    # Full-size property check on the last step's output: X_T from the
-    # selected inversion against the tip of the solution A^{-1} E_t.
+    # One more FP64 run of the three routines on the same inputs (checked
+    # calls), whose outputs go to the host for the oracle comparison; the
+    # solves are timed here (N then C, r = 1 and 64), and the residual
+    # ||A x - B|| / ||B|| is formed on the GPU from the original blocks.
     extra = {}
+    gpu_out = {}
+    from serinv_amd.logdet import pobta_logdet
+    from tests._blockprod import bta_matvec
+
+    algs_config["check_info"] = True
+    restore()
+    algs.pobtaf(Dw, Lw, Rw, Tw)
+    torch.cuda.synchronize()
+    gpu_out["L"] = [x.cpu().numpy() for x in (Dw, Lw, Rw, Tw)]
+    gpu_out["logdet"] = float(pobta_logdet(Dw, Tw))
     if not args.no_solve:
-        algs_config["check_info"] = True
-        restore()
-        algs.pobtaf(Dw, Lw, Rw, Tw)
-        torch.cuda.synchronize()
         sv = {}
         for r in (1, 64):
             Bw = B0[:, :r].contiguous().clone()
@@ -382,25 +436,18 @@ def main():
             t = e0.elapsed_time(e1)
             sv[r] = (t, Bw)
             extra[f"pobtas_ms_r{r}"] = round(t, 3)
+            res = float((bta_matvec((D0, L0, R0, T0), Bw) - B0[:, :r]).norm() / B0[:, :r].norm())
+            extra[f"solve_rel_residual_r{r}"] = float(f"{res:.3e}")
+            gpu_out[f"x_r{r}"] = Bw.cpu().numpy()
         byts = 2 * bta_factor_bytes(n, b, a)
         extra["pobtas_GBs_r1"] = round(byts / (sv[1][0] * 1e-3) * 1e-9, 1)
-        # residual || A x - B || / ||B|| via block matvec on the original blocks
-        x = sv[64][1]
-        Ax = torch.zeros_like(x)
-        xb = x[: n * b].view(n, b, -1)
-        Ax[: n * b].view(n, b, -1).copy_(torch.bmm(D0, xb))
-        if n > 1:
-            Ax[b: n * b].view(n - 1, b, -1).add_(torch.bmm(L0, xb[:-1]))
-            Ax[: (n - 1) * b].view(n - 1, b, -1).add_(torch.bmm(L0.transpose(1, 2).contiguous(), xb[1:]))
-        if a:
-            xt = x[n * b:]
-            Rflat = R0.permute(1, 0, 2).contiguous().view(a, n * b)
-            Ax[n * b:] += Rflat @ x[: n * b] + T0 @ xt
-            Ax[: n * b].view(n, b, -1).add_(torch.bmm(R0.transpose(1, 2).contiguous(), xt.expand(n, a, -1).contiguous()))
-        res = float((Ax - B0).norm() / B0.norm())
-        extra["solve_rel_residual_r64"] = float(f"{res:.3e}")
         print(f"[bench] pobtas r=1 {extra['pobtas_ms_r1']} ms, r=64 {extra['pobtas_ms_r64']} ms, "
-              f"residual {res:.2e}", file=sys.stderr, flush=True)
+              f"residuals {extra['solve_rel_residual_r1']:.2e} {extra['solve_rel_residual_r64']:.2e}",
+              file=sys.stderr, flush=True)
+        del sv
+    algs.pobtasi(Dw, Lw, Rw, Tw)
+    torch.cuda.synchronize()
+    gpu_out["X"] = [x.cpu().numpy() for x in (Dw, Lw, Rw, Tw)]
 
     # Roofline of the dominant kernel: pobtaf is one launch of the persistent
     # chain kernel (plus two flag fills of a few microseconds), so its event
@@ -456,8 +503,29 @@ def main():
         print(f"[bench] batched {out['batched']['batched_ms']} ms vs loop {out['batched']['loop_ms']} ms",
               file=sys.stderr, flush=True)
     if not args.no_cpu_baseline:
-        print("[bench] timing the CPU baseline ...", file=sys.stderr, flush=True)
-        out["cpu_baseline"] = cpu_baseline(b, a, n, args.cpu_seconds)
+        print("[bench] CPU-1 and CPU-2: the oracle on the host ...", file=sys.stderr, flush=True)
+        model, blas, npver = _cpu_info()
+        cpu1 = cpu1_baseline()
+        t_af, t_si, t_sv, threads, errs = cpu2_oracle(host, gpu_out, n, b, a)
+        out["cpu_baseline"] = {
+            "value": round((t_af + t_si) * 1e3, 1),
+            "unit": "ms (pobtaf+pobtasi, full workload, one run)",
+            "cores": threads,
+            "kind": "port",
+            "sample": f"oracle/serinv_np.py on the whole workload n={n} b={b} a={a} (CPU-2, one run, "
+                      f"not scaled): pobtaf {t_af:.2f} s, pobtasi {t_si:.2f} s, pobtas N+C r=1 "
+                      f"{t_sv.get(1, 0):.2f} s, r=64 {t_sv.get(64, 0):.2f} s; {model}; BLAS {blas}; "
+                      f"numpy {npver}",
+            "tflops": round((f_af + f_si) / (t_af + t_si) * 1e-12, 4),
+            "cpu1": cpu1,
+        }
+        out["parity_vs_oracle"] = {
+            "metric": "max|Y - Y_ref| / max|Y_ref| per output array (logdet: relative difference); "
+                      "Y_ref from oracle/serinv_np.py on the same inputs (CPU-2)",
+            **errs,
+        }
+        print(f"[bench] CPU-2 pobtaf {t_af:.2f} s pobtasi {t_si:.2f} s; GPU vs oracle {errs}",
+              file=sys.stderr, flush=True)
     else:
         out["cpu_baseline"] = None
     print(json.dumps(out), flush=True)

@@ -39,6 +39,7 @@ extern "C" {
 #define ROCBTA_OP_POBTASI 1
 #define ROCBTA_OP_POBTAS 2
 #define ROCBTA_OP_RESIDUAL 3
+#define ROCBTA_OP_LOGDET 4
 
 /* Library version string. */
 const char* rocbta_version(void);
@@ -78,9 +79,12 @@ int rocbta_dpobtas(int64_t n, int64_t b, int64_t a, const double* D, const doubl
 
 /* log det A = 2 sum log diag(L_ii) + 2 sum log diag(L_T) from the factor
  * written by rocbta_dpobtaf; result stored to the device double *out.
- * No counterpart in the reference (extension for likelihood evaluation). */
+ * Workspace: rocbta_workspace_bytes(ROCBTA_OP_LOGDET, n, b, a, 0) bytes of
+ * partial sums (the sum is deterministic: partials are added in a fixed
+ * order).  No counterpart in the reference (extension for likelihood
+ * evaluation). */
 int rocbta_dlogdet(int64_t n, int64_t b, int64_t a, const double* D, const double* T,
-                   double* out, void* stream);
+                   double* out, void* workspace, size_t workspace_bytes, void* stream);
 
 /* Mixed-precision factorization / selected inversion (no counterpart in the
  * reference).  Same arguments, layout, workspace and results as

@@ -15,7 +15,7 @@ ROCBTA_OK = 0
 ROCBTA_ERR_ARG = -1
 ROCBTA_ERR_HIP = -2
 ROCBTA_ERR_WORKSPACE = -3
-OP_POBTAF, OP_POBTASI, OP_POBTAS, OP_RESIDUAL = 0, 1, 2, 3
+OP_POBTAF, OP_POBTASI, OP_POBTAS, OP_RESIDUAL, OP_LOGDET = 0, 1, 2, 3, 4
 BATCHED_FACTOR, BATCHED_SOLVE, BATCHED_SELINV = 1, 2, 4
 
 _ERRORS = {
@@ -39,7 +39,7 @@ SIGNATURES = {
                                  _c_size, _c_p]),
     "rocbta_dpobtas": (_c_int, [_c_i64, _c_i64, _c_i64, _c_p, _c_p, _c_p, _c_p, _c_p, _c_i64,
                                 _c_i64, ctypes.c_char, _c_int, _c_p, _c_size, _c_p]),
-    "rocbta_dlogdet": (_c_int, [_c_i64, _c_i64, _c_i64, _c_p, _c_p, _c_p, _c_p]),
+    "rocbta_dlogdet": (_c_int, [_c_i64, _c_i64, _c_i64, _c_p, _c_p, _c_p, _c_p, _c_size, _c_p]),
     "rocbta_dag_selfcheck": (_c_i64, [_c_i64, _c_i64, _c_i64]),
     "rocbta_mpobtaf": (_c_int, [_c_i64, _c_i64, _c_i64, _c_p, _c_p, _c_p, _c_p, _c_int, _c_p,
                                 _c_p, _c_size, _c_p]),

@@ -5,7 +5,7 @@ the SPD path: same names, argument order, in-place block layout, keyword
 arguments and error types, on float64 torch tensors on a ROCm device.
 """
 
-from ._common import config
+from ._common import check_status, config
 from .pobtaf import pobtaf
 from .pobtas import pobtas
 from .pobtasi import pobtasi
@@ -13,4 +13,4 @@ from .pobtf import pobtf
 from .pobts import pobts
 from .pobtsi import pobtsi
 
-__all__ = ["pobtaf", "pobtasi", "pobtas", "pobtf", "pobtsi", "pobts", "config"]
+__all__ = ["pobtaf", "pobtasi", "pobtas", "pobtf", "pobtsi", "pobts", "config", "check_status"]

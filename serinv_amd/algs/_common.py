@@ -6,6 +6,8 @@ reference (src/serinv/algs/*.py).  There is no host path and no fallback: a
 tensor that is not on the GPU, or a missing librocbta.so, raises.
 """
 
+import threading
+
 import numpy as np
 import torch
 
@@ -15,8 +17,45 @@ from .. import _lib
 # unchanged).  check_info: after a factorization, read the device pivot status
 # back and raise numpy.linalg.LinAlgError on a non-positive pivot, as the
 # reference's NumPy path does (scipy.linalg.cholesky raises).  Turning it off
-# keeps the call fully asynchronous, as the reference's CuPy path is.
+# keeps the call fully asynchronous, as the reference's CuPy path is; the
+# status words of such calls (pivot failures and internal scheduling
+# time-outs, after which the outputs are partial) stay on the device and
+# check_status() reports them later.
 config = {"check_info": True}
+
+_pending = []  # (device int32 status tensor, routine name) of unchecked calls
+_pending_lock = threading.Lock()
+_PENDING_MAX = 256
+
+
+def note_status(status, what):
+    """Keep the device status word of an unchecked call for check_status()."""
+    with _pending_lock:
+        _pending.append((status.reshape(-1)[:1].clone(), what))
+        if len(_pending) > _PENDING_MAX:
+            # fold into two entries (smallest and largest status) without a sync
+            allv = torch.cat([p[0].to(torch.int64) for p in _pending])
+            names = sorted({p[1] for p in _pending})
+            _pending[:] = [(allv.min().reshape(1).to(torch.int32), "/".join(names)),
+                           (allv.max().reshape(1).to(torch.int32), "/".join(names))]
+
+
+def check_status():
+    """Synchronize on and clear the status words of every call made with
+    config["check_info"] = False, and raise like a checked call would on the
+    first failure: RocbtaError after an internal scheduling time-out (the
+    outputs of that call are then partial), numpy.linalg.LinAlgError after a
+    non-positive pivot."""
+    with _pending_lock:
+        items = list(_pending)
+        _pending.clear()
+    for st, what in items:
+        v = int(st.item())
+        if v < 0:
+            raise _lib.RocbtaError(f"{what}: internal scheduling time-out (outputs incomplete)")
+        if v > 0:
+            raise np.linalg.LinAlgError(
+                f"{what}: the leading minor of order {v} is not positive definite")
 
 
 def check_kwargs(kwargs, expected):

@@ -8,8 +8,8 @@ hand-written gfx950 HIP kernels (librocbta.so, rocbta_dpobtaf).
 import torch
 
 from .. import _lib
-from ._common import (bta_shapes, check_kwargs, config, ptr, raise_on_info, reject_unsupported,
-                      stream_of, workspace)
+from ._common import (bta_shapes, check_kwargs, config, note_status, ptr, raise_on_info,
+                      reject_unsupported, stream_of, workspace)
 
 
 def pobtaf(A_diagonal_blocks, A_lower_diagonal_blocks, A_lower_arrow_blocks, A_arrow_tip_block,
@@ -44,3 +44,5 @@ def _factor(D, L, R, T, n, b, a, dev, factorize_last_block, what, mixed=False):
     _lib.check(rc, what)
     if config["check_info"]:
         raise_on_info(info, what)
+    else:
+        note_status(info, what)

@@ -8,7 +8,7 @@ not validate its keyword arguments.
 import torch
 
 from .. import _lib
-from ._common import _tensor, bta_shapes, config, ptr, stream_of, workspace
+from ._common import _tensor, bta_shapes, config, note_status, ptr, stream_of, workspace
 
 
 def pobtas(L_diagonal_blocks, L_lower_diagonal_blocks, L_lower_arrow_blocks, L_arrow_tip_block, B,
@@ -53,8 +53,10 @@ def _solve(D, L, R, T, B, trans, kwargs, n, b, a, dev, what):
     rc = lib.rocbta_dpobtas(n, b, a, ptr(D), ptr(L), ptr(R), ptr(T), ptr(B), r, ldb,
                             trans.encode(), int(partial), ws.data_ptr(), nbytes, stream_of(dev))
     _lib.check(rc, what)
-    if config["check_info"] and not partial:
-        # the persistent sweep leaves its status in the first int32 of the
-        # workspace (0, or -1 after an internal scheduling time-out)
-        if int(ws[:4].view(torch.int32).item()) < 0:
-            raise _lib.RocbtaError(f"{what}: internal scheduling time-out in the solve kernel")
+    # the solve leaves its status in the first int32 of the workspace (0, or
+    # -1 after an internal scheduling time-out)
+    status = ws[:4].view(torch.int32)
+    if not config["check_info"]:
+        note_status(status, what)
+    elif int(status.item()) < 0:
+        raise _lib.RocbtaError(f"{what}: internal scheduling time-out in the solve kernel")

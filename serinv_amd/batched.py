@@ -55,6 +55,9 @@ class BatchedBTA:
                     raise ValueError(f"rhs[{k}] must be 1-D or 2-D")
                 if B.shape[0] != n * b + a or B.device != dev:
                     raise ValueError(f"rhs[{k}] must have {n * b + a} rows on the systems' device")
+                if r > 512:
+                    raise ValueError(f"rhs[{k}]: the batched plan solves at most 512 columns per system "
+                                     f"(got {r}); solve wider right-hand sides with pobtas")
                 rs.append(r)
                 ptrs["B"].append(B.data_ptr())
                 self._keep.append(B)

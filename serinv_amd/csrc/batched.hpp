@@ -42,8 +42,13 @@ struct BatchedPlan {
   char* desc = nullptr;  // descriptor lists
   // factorization
   PobtafChain* d_chains = nullptr;
-  int* d_gclaim = nullptr;  // [0] factor, [1] forward sweep, [2] backward sweep
-  int* d_cu_busy = nullptr;
+  // The matrices run in waves of consecutive matrices, each wave one
+  // persistent launch with at most max_chains chain workgroups, so that
+  // helpers always have free CUs (factorization) or resident slots (sweeps)
+  // whatever the batch size.  fwave[w] / swave[v]: first matrix of a wave.
+  std::vector<int> fwave, swave;
+  int* d_gclaim = nullptr;  // factor wave w: [w]; sweep wave v: [nfw + 2v + dir]
+  int* d_cu_busy = nullptr;  // 4096 ints per factor wave
   int* d_info = nullptr;
   int* d_status = nullptr;
   FillOp* d_zero = nullptr;
@@ -132,8 +137,9 @@ int record_pobtasi(int64_t n, int64_t b, int64_t a, double* D, double* L, double
 }
 
 template <int RC>
-int launch_sweep_multi(const BatchedPlan& p, const SweepP* Ps, int* gclaim, hipStream_t s) {
-  pobtas_sweep_multi_kernel<RC><<<p.sweep_grid, SW_THREADS, 0, s>>>(Ps, p.d_chain_begin, p.nmat, gclaim);
+int launch_sweep_multi(const BatchedPlan& p, const SweepP* Ps, const int* chain_begin, int nm,
+                       int* gclaim, hipStream_t s) {
+  pobtas_sweep_multi_kernel<RC><<<p.sweep_grid, SW_THREADS, 0, s>>>(Ps, chain_begin, nm, gclaim);
   RB_CHECK(hipGetLastError());
   return ROCBTA_OK;
 }
@@ -142,22 +148,31 @@ int batched_enqueue(const BatchedPlan& p, hipStream_t s) {
   const int64_t gx = std::min<int64_t>(1024, (p.maxzero + 255) / 256);
   fill_list_kernel<<<dim3((unsigned)std::max<int64_t>(gx, 1), (unsigned)p.nzero), 256, 0, s>>>(p.d_zero);
   RB_CHECK(hipGetLastError());
-  pobtaf_chain_multi_kernel<<<p.chain_grid, CH_THREADS, 0, s>>>(p.d_chains, p.nmat, p.d_gclaim,
-                                                                 p.d_cu_busy);
-  RB_CHECK(hipGetLastError());
+  const int nfw = (int)p.fwave.size() - 1;
+  for (int w = 0; w < nfw; ++w) {
+    const int m0 = p.fwave[w], nm = p.fwave[w + 1] - m0;
+    pobtaf_chain_multi_kernel<<<std::max(nm + 1, p.chain_grid), CH_THREADS, 0, s>>>(
+        p.d_chains + m0, nm, p.d_gclaim + w, p.d_cu_busy + (int64_t)w * 4096);
+    RB_CHECK(hipGetLastError());
+  }
   logdet_multi_kernel<<<p.nmat, 256, 0, s>>>(p.d_logdet);
   RB_CHECK(hipGetLastError());
   if (p.ops & ROCBTA_BATCHED_SOLVE) {
+    const int nsw = (int)p.swave.size() - 1;
     for (int dir = 0; dir < 2; ++dir) {
       fill_u64_kernel<<<(int)std::min<int64_t>(4096, (p.nempty + 255) / 256), 256, 0, s>>>(
           p.d_empty, p.nempty, SW_EMPTY);
       RB_CHECK(hipGetLastError());
-      const SweepP* Ps = dir == 0 ? p.d_fwd : p.d_bwd;
-      int* gc = p.d_gclaim + 1 + dir;
-      int rc = p.sweep_rc == 1 ? launch_sweep_multi<1>(p, Ps, gc, s)
-               : p.sweep_rc == 4 ? launch_sweep_multi<4>(p, Ps, gc, s)
-                                 : launch_sweep_multi<8>(p, Ps, gc, s);
-      if (rc) return rc;
+      for (int v = 0; v < nsw; ++v) {
+        const int m0 = p.swave[v], nm = p.swave[v + 1] - m0;
+        const SweepP* Ps = (dir == 0 ? p.d_fwd : p.d_bwd) + m0;
+        const int* cb = p.d_chain_begin + m0 + v;  // per wave: nm + 1 relative entries
+        int* gc = p.d_gclaim + nfw + 2 * v + dir;
+        int rc = p.sweep_rc == 1 ? launch_sweep_multi<1>(p, Ps, cb, nm, gc, s)
+                 : p.sweep_rc == 4 ? launch_sweep_multi<4>(p, Ps, cb, nm, gc, s)
+                                   : launch_sweep_multi<8>(p, Ps, cb, nm, gc, s);
+        if (rc) return rc;
+      }
     }
   }
   if (p.ops & ROCBTA_BATCHED_SELINV) {
@@ -212,6 +227,8 @@ int rocbta_batched_create(int nmat, const int64_t* n, const int64_t* b, const in
     if (solve && (r[m] < 1 || !B[m])) return ROCBTA_ERR_ARG;
     if (solve) rmax = std::max(rmax, r[m]);
   }
+  // one matrix's column chunks must fit one sweep wave
+  if (solve && cdiv(rmax, sweep_rc(rmax)) > SW_MAX_CHAINS) return ROCBTA_ERR_ARG;
   BatchedPlan* p = new BatchedPlan();
   p->nmat = nmat;
   p->ops = ops;
@@ -234,8 +251,28 @@ int rocbta_batched_create(int nmat, const int64_t* n, const int64_t* b, const in
     int nt, ta, NT, NG, nchunks;
   };
   std::vector<MatOff> mo(nmat);
-  const size_t o_gclaim = take(4 * sizeof(int));
-  const size_t o_cu = take(4096 * sizeof(int));
+  // waves: at most a quarter of the CUs run factorization chains at once,
+  // and a sweep launch has at most SW_MAX_CHAINS chains
+  const int cus = device_cus();
+  const int fcap = std::max(1, cus / 4);
+  for (int m = 0; m < nmat; m += fcap) p->fwave.push_back(m);
+  p->fwave.push_back(nmat);
+  if (solve) {
+    int chains = 0;
+    p->swave.push_back(0);
+    for (int m = 0; m < nmat; ++m) {
+      const int c = cdiv(r[m], RC);
+      if (chains + c > SW_MAX_CHAINS) {
+        p->swave.push_back(m);
+        chains = 0;
+      }
+      chains += c;
+    }
+    p->swave.push_back(nmat);
+  }
+  const int nfw = (int)p->fwave.size() - 1, nsw = solve ? (int)p->swave.size() - 1 : 0;
+  const size_t o_gclaim = take((size_t)(nfw + 2 * nsw) * sizeof(int));
+  const size_t o_cu = take((size_t)nfw * 4096 * sizeof(int));
   const size_t o_info = take((size_t)nmat * sizeof(int));
   const size_t o_status = take((size_t)(1 + 2 * nmat) * sizeof(int));  // status, 2 counters per matrix
   const size_t o_ld = take((size_t)nmat * sizeof(double));
@@ -285,7 +322,7 @@ int rocbta_batched_create(int nmat, const int64_t* n, const int64_t* b, const in
   std::vector<PobtafChain> chains(nmat);
   std::vector<LogdetDesc> lds(nmat);
   std::vector<SweepP> fwd, bwd;
-  std::vector<int> chain_begin(nmat + 1, 0);
+  std::vector<int> chain_begin;  // per sweep wave: nm + 1 entries, relative to the wave
   std::vector<FillOp> zero;
   zero.push_back({(double*)(base + o_gclaim), (int64_t)((o_ld - o_gclaim) / sizeof(double)), 0.0});
   for (int m = 0; m < nmat; ++m) {
@@ -329,14 +366,20 @@ int rocbta_batched_create(int nmat, const int64_t* n, const int64_t* b, const in
       P.trans = 1;
       P.counter = p->d_status + 2 + 2 * m;
       bwd.push_back(P);
-      chain_begin[m + 1] = chain_begin[m] + q.nchunks;
     }
+  }
+  for (int v = 0; solve && v + 1 < (int)p->swave.size(); ++v) {
+    int c = 0;
+    for (int m = p->swave[v]; m < p->swave[v + 1]; ++m) {
+      chain_begin.push_back(c);
+      c += mo[m].nchunks;
+    }
+    chain_begin.push_back(c);
   }
   p->nzero = (int)zero.size();
   for (const FillOp& z : zero) p->maxzero = std::max(p->maxzero, z.count);
-  const int cus = device_cus();
-  p->chain_grid = std::max(nmat + 1, 2 * cus);
-  p->sweep_grid = std::max((solve ? chain_begin[nmat] : 0) + 1, 2 * cus);
+  p->chain_grid = std::max(fcap + 1, 2 * cus);
+  p->sweep_grid = std::max(SW_MAX_CHAINS + 1, 2 * cus);
 
   // ---- selected inversion: record every matrix's op sequence
   Recorder rec;
@@ -369,7 +412,7 @@ int rocbta_batched_create(int nmat, const int64_t* n, const int64_t* b, const in
   const size_t d_zero = dtake(zero.size() * sizeof(FillOp));
   const size_t d_fwd = dtake(fwd.size() * sizeof(SweepP) + 1);
   const size_t d_bwd = dtake(bwd.size() * sizeof(SweepP) + 1);
-  const size_t d_cb = dtake((nmat + 1) * sizeof(int));
+  const size_t d_cb = dtake((chain_begin.size() + 1) * sizeof(int));
   for (Phase& ph : rec.phases) {
     PhaseLaunch pl;
     pl.nfill = (int)ph.fills.size();
@@ -406,7 +449,7 @@ int rocbta_batched_create(int nmat, const int64_t* n, const int64_t* b, const in
   put(d_zero, zero.data(), zero.size() * sizeof(FillOp));
   put(d_fwd, fwd.data(), fwd.size() * sizeof(SweepP));
   put(d_bwd, bwd.data(), bwd.size() * sizeof(SweepP));
-  put(d_cb, chain_begin.data(), (nmat + 1) * sizeof(int));
+  put(d_cb, chain_begin.data(), chain_begin.size() * sizeof(int));
   for (size_t k = 0; k < rec.phases.size(); ++k) {
     const Phase& ph = rec.phases[k];
     const PhaseLaunch& pl = p->phases[k];
@@ -435,11 +478,9 @@ int rocbta_batched_create(int nmat, const int64_t* n, const int64_t* b, const in
 int rocbta_batched_set_grid(rocbta_batched_plan plan, int factor_grid, int solve_grid) {
   BatchedPlan* p = (BatchedPlan*)plan;
   if (!p) return ROCBTA_ERR_ARG;
-  int solve_chains = 0;
-  if (p->ops & ROCBTA_BATCHED_SOLVE)
-    for (int m = 0; m < p->nmat; ++m) solve_chains += cdiv(p->r[m], p->sweep_rc);
-  if (factor_grid > 0) p->chain_grid = std::max(p->nmat + 1, factor_grid);
-  if (solve_grid > 0) p->sweep_grid = std::max(solve_chains + 1, solve_grid);
+  // every launch gets at least one workgroup more than its chains
+  if (factor_grid > 0) p->chain_grid = factor_grid;
+  if (solve_grid > 0) p->sweep_grid = std::max(SW_MAX_CHAINS + 1, solve_grid);
   if (p->exec) {  // the captured graph holds the old launch shapes
     (void)hipGraphExecDestroy(p->exec);
     p->exec = nullptr;
@@ -487,7 +528,8 @@ int rocbta_batched_results(rocbta_batched_plan plan, int* info, double* logdet, 
 int rocbta_batched_launch_count(rocbta_batched_plan plan) {
   BatchedPlan* p = (BatchedPlan*)plan;
   if (!p) return -1;
-  int k = 3 + ((p->ops & ROCBTA_BATCHED_SOLVE) ? 4 : 0);
+  const int nfw = (int)p->fwave.size() - 1, nsw = (int)p->swave.size() - 1;
+  int k = 2 + nfw + ((p->ops & ROCBTA_BATCHED_SOLVE) ? 2 * (1 + nsw) : 0);
   for (const PhaseLaunch& ph : p->phases)
     k += (ph.nfill > 0) + (ph.ntile > 0) + (ph.ngemm > 0 && ph.gemm_tiles > 0);
   return k;

@@ -18,9 +18,11 @@
 #include "pobtaf_chain.hpp"
 #include "solve_f64.hpp"
 #include "solve_sweep.hpp"
+#include "solve_block.hpp"
 #include "resid_f64.hpp"
 #include "tile_f64.hpp"
 
+#include <atomic>
 #include <cstdlib>
 #include <cstring>
 #include <mutex>
@@ -401,29 +403,70 @@ int block_trtri(const double* Ls, int64_t lds, int64_t sL, double* X, int64_t ld
 }
 
 // -------------------------------------------------------- workspace layout
-// Per-device side stream (default priority) with a growable event pool,
-// used to overlap independent precompute with the caller's stream.
+// Side streams (default priority) with growable event pools, used to
+// overlap independent precompute with the caller's stream.  A call checks
+// one out of its device's pool for its whole duration and returns it at the
+// end (SideLease), so concurrent calls from several host threads never
+// record or wait on each other's events.  Reuse after a call is safe: a
+// stream wait binds to the event record that precedes it at enqueue time.
 constexpr int SI_CHUNK = 16;
 struct SideStream {
   hipStream_t s = nullptr;
   std::vector<hipEvent_t> ev;
 };
 
-SideStream* side_stream(int nevents) {
-  static std::mutex mu;
-  static SideStream per_dev[64];
-  int dev = 0;
-  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return nullptr;
-  std::lock_guard<std::mutex> lock(mu);
-  SideStream& ss = per_dev[dev];
-  if (!ss.s && hipStreamCreateWithFlags(&ss.s, hipStreamNonBlocking) != hipSuccess) return nullptr;
-  while ((int)ss.ev.size() < nevents) {
-    hipEvent_t e;
-    if (hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess) return nullptr;
-    ss.ev.push_back(e);
-  }
-  return &ss;
+struct SidePool {
+  std::mutex mu;
+  std::vector<SideStream*> free_list[64];
+};
+SidePool& side_pool() {
+  static SidePool pool;  // streams and events live until the process ends
+  return pool;
 }
+
+class SideLease {
+ public:
+  SideLease(int nevents) {
+    if (hipGetDevice(&dev_) != hipSuccess || dev_ < 0 || dev_ >= 64) return;
+    SidePool& pool = side_pool();
+    {
+      std::lock_guard<std::mutex> lock(pool.mu);
+      if (!pool.free_list[dev_].empty()) {
+        ss_ = pool.free_list[dev_].back();
+        pool.free_list[dev_].pop_back();
+      }
+    }
+    if (!ss_) {
+      ss_ = new SideStream();
+      if (hipStreamCreateWithFlags(&ss_->s, hipStreamNonBlocking) != hipSuccess) {
+        delete ss_;
+        ss_ = nullptr;
+        return;
+      }
+    }
+    while ((int)ss_->ev.size() < nevents) {
+      hipEvent_t e;
+      if (hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess) {
+        release();
+        return;
+      }
+      ss_->ev.push_back(e);
+    }
+  }
+  ~SideLease() { release(); }
+  SideStream* get() const { return ss_; }
+
+ private:
+  void release() {
+    if (!ss_) return;
+    SidePool& pool = side_pool();
+    std::lock_guard<std::mutex> lock(pool.mu);
+    pool.free_list[dev_].push_back(ss_);
+    ss_ = nullptr;
+  }
+  int dev_ = -1;
+  SideStream* ss_ = nullptr;
+};
 
 
 struct SiLayout {
@@ -487,16 +530,16 @@ AfLayout af_layout(int64_t n, int64_t b, int64_t a) {
 }
 
 int device_cus() {
-  static int cus[64] = {0};
+  static std::atomic<int> cus[64];  // zero-initialised (static storage)
   int dev = 0;
   if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return 256;
-  if (cus[dev] == 0) {
-    int v = 0;
+  int v = cus[dev].load(std::memory_order_relaxed);
+  if (v == 0) {
     if (hipDeviceGetAttribute(&v, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || v <= 0)
       v = 256;
-    cus[dev] = v;
+    cus[dev].store(v, std::memory_order_relaxed);
   }
-  return cus[dev];
+  return v;
 }
 
 // ROCBTA_POBTAF_ALGO=loop forces the launch-per-tile path (used by tests to
@@ -558,6 +601,12 @@ struct SwLayout {
 
 int sweep_rc(int64_t r) { return r <= 1 ? 1 : (r <= 4 ? 4 : 8); }
 
+// At most this many column chunks (chain workgroups) per sweep launch.  A
+// chain waits for helper partial sums, so chains may take only a fraction of
+// the resident workgroup slots (two sweep workgroups fit per CU); wider
+// right-hand sides run as several launches over groups of columns.
+constexpr int SW_MAX_CHAINS = 64;
+
 SwLayout sw_layout(int64_t n, int64_t b, int64_t a, int64_t r) {
   SwLayout l{};
   l.nt = cdiv(b, SW_T);
@@ -565,7 +614,7 @@ SwLayout sw_layout(int64_t n, int64_t b, int64_t a, int64_t r) {
   l.NT = (int)(n * l.nt);
   l.NG = l.NT + l.ta;
   l.rc = sweep_rc(r);
-  l.nchunks = std::max(1, cdiv(r, l.rc));
+  l.nchunks = std::min(SW_MAX_CHAINS, std::max(1, cdiv(r, l.rc)));  // per launch
   l.ntasks = 2 * l.NT * l.nchunks;
   size_t off = 0;
   auto take = [&](size_t bytes) {
@@ -592,11 +641,26 @@ __global__ void fill_u64_kernel(unsigned long long* p, int64_t count, unsigned l
   for (; i < count; i += stride) p[i] = v;
 }
 
-// ROCBTA_POBTAS_ALGO=loop forces the launch-per-tile solve (tests cover both).
-bool pobtas_use_sweep(int partial) {
+// Solve algorithm: the block-level sweep (precomputed diagonal-block
+// inverses, two dependent device-wide products per block row) for large
+// blocks, the 64-row-tile sweep otherwise, the launch-per-tile form for
+// partial solves on small blocks.  ROCBTA_POBTAS_ALGO = block | sweep | loop
+// forces one (tests cover all three; block falls back when the shape does
+// not fit it).
+enum SolveAlgo { SOLVE_LOOP = 0, SOLVE_SWEEP = 1, SOLVE_BLOCK = 2 };
+constexpr int BS_MIN_B = 256;   // default: block sweep from this block size on
+constexpr int BS_MAX_G = 1024;  // workgroups of the block sweep (tip partial sums sized for it)
+
+int bs_cw(int64_t r) { return r <= 1 ? 1 : (r <= 4 ? 4 : 8); }
+bool bs_fits(int64_t a, int64_t r) { return a * bs_cw(r) <= BS_TIPMAX; }
+
+int pobtas_algo(int64_t b, int64_t a, int64_t r, int partial) {
   const char* e = getenv("ROCBTA_POBTAS_ALGO");
-  if (e && strcmp(e, "loop") == 0) return false;
-  return !partial;
+  if (e && strcmp(e, "loop") == 0) return SOLVE_LOOP;
+  if (e && strcmp(e, "sweep") == 0) return partial ? SOLVE_LOOP : SOLVE_SWEEP;
+  if (e && strcmp(e, "block") == 0) return bs_fits(a, r) ? SOLVE_BLOCK : (partial ? SOLVE_LOOP : SOLVE_SWEEP);
+  if (b >= BS_MIN_B && bs_fits(a, r)) return SOLVE_BLOCK;
+  return partial ? SOLVE_LOOP : SOLVE_SWEEP;
 }
 
 // Inverses of the diagonal tiles of an m_total x m_total lower-triangular
@@ -665,22 +729,135 @@ int pobtas_sweep(int64_t n, int64_t b, int64_t a, const double* D, const double*
     rc = diag_tile_inverses(T, a, 0, (int)a, 1, tinv + (int64_t)lay.NT * SW_T * SW_T, 0, s);
     if (rc) return rc;
   }
-  const int64_t nflag = (int64_t)((lay.tinv - lay.status) / sizeof(int));
-  fill_i32_kernel<<<(int)std::min<int64_t>(1024, (nflag + 255) / 256), 256, 0, s>>>(P.status, nflag, 0);
+  set_int_kernel<<<1, 64, 0, s>>>(P.status, 0);  // kept across the column groups
   RB_CHECK(hipGetLastError());
+  const int64_t group_cols = (int64_t)lay.nchunks * lay.rc;
   const int64_t nempty = (int64_t)((lay.total - lay.ybuf) / sizeof(double));
-  fill_u64_kernel<<<(int)std::min<int64_t>(4096, (nempty + 255) / 256), 256, 0, s>>>(
-      (unsigned long long*)(ws + lay.ybuf), nempty, SW_EMPTY);
-  RB_CHECK(hipGetLastError());
-  const int grid = std::max(P.nchunks + 1, std::min(P.nchunks + P.ntasks, 2 * device_cus()));
-  switch (lay.rc) {
-    case 1:
-      return launch_sweep<1>(P, grid, s);
-    case 4:
-      return launch_sweep<4>(P, grid, s);
-    default:
-      return launch_sweep<8>(P, grid, s);
+  for (int64_t c0 = 0; c0 < r; c0 += group_cols) {
+    P.B = B + c0;
+    P.r = (int)std::min<int64_t>(group_cols, r - c0);
+    P.nchunks = cdiv(P.r, lay.rc);
+    P.ntasks = 2 * lay.NT * P.nchunks;
+    // claim ticket and helper queue head (the slots between them included)
+    fill_i32_kernel<<<1, 256, 0, s>>>(P.claim, (int64_t)((lay.tinv - lay.claim) / sizeof(int)), 0);
+    RB_CHECK(hipGetLastError());
+    fill_u64_kernel<<<(int)std::min<int64_t>(4096, (nempty + 255) / 256), 256, 0, s>>>(
+        (unsigned long long*)(ws + lay.ybuf), nempty, SW_EMPTY);
+    RB_CHECK(hipGetLastError());
+    const int grid = std::max(P.nchunks + 1, std::min(P.nchunks + P.ntasks, 2 * device_cus()));
+    switch (lay.rc) {
+      case 1:
+        rc = launch_sweep<1>(P, grid, s);
+        break;
+      case 4:
+        rc = launch_sweep<4>(P, grid, s);
+        break;
+      default:
+        rc = launch_sweep<8>(P, grid, s);
+        break;
+    }
+    if (rc) return rc;
   }
+  return ROCBTA_OK;
+}
+
+struct BsLayout {
+  size_t status, linv, w, tinv, wt, sbuf, tbuf, tpart, tipbuf, total;
+  int cw;
+};
+
+BsLayout bs_layout(int64_t n, int64_t b, int64_t a, int64_t r) {
+  BsLayout l{};
+  l.cw = bs_cw(r);
+  size_t off = 0;
+  auto take = [&](size_t bytes) {
+    size_t o = off;
+    off = align256(off + bytes);
+    return o;
+  };
+  const int64_t ntb = (b + TT - 1) / TT, nta = (a + TT - 1) / TT;
+  l.status = take(sizeof(int));
+  l.linv = take((size_t)n * b * b * sizeof(double));
+  // trtri scratch: 64-wide levels need ntb * 64^2 per block, halving b^2 / 4
+  l.w = take((size_t)n * std::max<int64_t>(ntb * TT * TT, ntb * TT * ntb * TT / 4) * sizeof(double));
+  l.tinv = take((size_t)a * a * sizeof(double));
+  l.wt = take((size_t)nta * TT * TT * sizeof(double));
+  l.sbuf = take((size_t)n * b * l.cw * sizeof(double));
+  l.tbuf = take((size_t)n * b * l.cw * sizeof(double));
+  l.tpart = take((size_t)BS_MAX_G * a * l.cw * sizeof(double));
+  l.tipbuf = take((size_t)a * l.cw * sizeof(double));
+  l.total = off;
+  return l;
+}
+
+template <int CW>
+int launch_block_sweep(const BlockSweepP& P, hipStream_t s) {
+  pobtas_block_kernel<CW><<<P.G, BS_THREADS, 0, s>>>(P);
+  RB_CHECK(hipGetLastError());
+  return ROCBTA_OK;
+}
+
+// pobtas with the block-level sweep: the inverses of the diagonal blocks and
+// of the tip factor first (batched trtri), then one persistent launch per
+// group of at most CW right-hand-side columns.
+int pobtas_block(int64_t n, int64_t b, int64_t a, const double* D, const double* L,
+                 const double* R, const double* T, double* B, int64_t r, int64_t ldb, int trans,
+                 int partial, char* ws, hipStream_t s) {
+  const BsLayout lay = bs_layout(n, b, a, r);
+  int* status = (int*)(ws + lay.status);
+  set_int_kernel<<<1, 64, 0, s>>>(status, 0);
+  RB_CHECK(hipGetLastError());
+  double* Linv = (double*)(ws + lay.linv);
+  double* Tinv = (double*)(ws + lay.tinv);
+  int rc = block_trtri(D, b, b * b, Linv, b, b * b, (int)b, (int)n, (double*)(ws + lay.w), s, true);
+  if (rc) return rc;
+  if (a > 0) {
+    rc = block_trtri(T, a, 0, Tinv, a, a * a, (int)a, 1, (double*)(ws + lay.wt), s);
+    if (rc) return rc;
+  }
+  BlockSweepP P{};
+  P.L = L;
+  P.R = R;
+  P.Linv = Linv;
+  P.Tinv = Tinv;
+  P.ldb = ldb;
+  P.n = (int)n;
+  P.b = (int)b;
+  P.a = (int)a;
+  P.trans = trans;
+  P.partial = partial;
+  P.G = (int)std::min<int64_t>(std::min<int64_t>((b + BS_RW - 1) / BS_RW, BS_MAX_G), device_cus());
+  P.sbuf = (double*)(ws + lay.sbuf);
+  P.tbuf = (double*)(ws + lay.tbuf);
+  P.tpart = (double*)(ws + lay.tpart);
+  P.tipbuf = (double*)(ws + lay.tipbuf);
+  P.status = status;
+  P.spin_limit = 1ll << 22;
+  const int64_t nempty = (int64_t)((lay.total - lay.sbuf) / sizeof(double));
+  for (int64_t c0 = 0; c0 < r; c0 += lay.cw) {
+    P.B = B + c0;
+    P.r = (int)std::min<int64_t>(lay.cw, r - c0);
+    fill_u64_kernel<<<(int)std::min<int64_t>(4096, (nempty + 255) / 256), 256, 0, s>>>(
+        (unsigned long long*)(ws + lay.sbuf), nempty, SW_EMPTY);
+    RB_CHECK(hipGetLastError());
+    switch (lay.cw) {
+      case 1:
+        rc = launch_block_sweep<1>(P, s);
+        break;
+      case 4:
+        rc = launch_block_sweep<4>(P, s);
+        break;
+      default:
+        rc = launch_block_sweep<8>(P, s);
+        break;
+    }
+    if (rc) return rc;
+  }
+  return ROCBTA_OK;
+}
+
+int logdet_parts(int64_t n, int64_t b, int64_t a) {
+  return (int)((n * b + a + LD_ROWS_PER_WG - 1) / LD_ROWS_PER_WG);
 }
 
 bool bad_shape(int64_t n, int64_t b, int64_t a) {
@@ -702,8 +879,16 @@ size_t rocbta_workspace_bytes(int op, int64_t n, int64_t b, int64_t a, int64_t r
       return af_layout(n, b, a).total;
     case ROCBTA_OP_POBTASI:
       return si_layout(n, b, a).total;
-    case ROCBTA_OP_POBTAS:
-      return sw_layout(n, b, a, std::max<int64_t>(r, 1)).total;
+    case ROCBTA_OP_POBTAS: {
+      // enough for the algorithm the shape selects (partial or not)
+      const int64_t rr = std::max<int64_t>(r, 1);
+      size_t need = sw_layout(n, b, a, rr).total;
+      if (pobtas_algo(b, a, rr, 0) == SOLVE_BLOCK || pobtas_algo(b, a, rr, 1) == SOLVE_BLOCK)
+        need = std::max(need, bs_layout(n, b, a, rr).total);
+      return need;
+    }
+    case ROCBTA_OP_LOGDET:
+      return align256((size_t)logdet_parts(n, b, a) * sizeof(double));
     case ROCBTA_OP_RESIDUAL: {
       const int64_t rr = std::max<int64_t>(r, 1), rc = sweep_rc(rr);
       return align256((size_t)cdiv(rr, rc) * n * cdiv(a, SW_T) * SW_T * rc * sizeof(double)) + 256;
@@ -816,7 +1001,8 @@ int rocbta_dpobtasi(int64_t n, int64_t b, int64_t a, double* D, double* L, doubl
   //   L_ii^{-1}; G_i = L_{i+1,i} L_ii^{-1}; Gt_i = L_{t,i} L_ii^{-1};
   //   S_i = L_ii^{-T} L_ii^{-1} -> D_i.
   const int nchunks = (npre + SI_CHUNK - 1) / SI_CHUNK;
-  SideStream* side = side_stream(nchunks + 1);
+  SideLease lease(nchunks + 1);
+  SideStream* side = lease.get();
   if (!side) return ROCBTA_ERR_HIP;
   // ROCBTA_SI_OVERLAP=0 runs the precompute on the caller's stream first.
   const char* ov = getenv("ROCBTA_SI_OVERLAP");
@@ -1015,7 +1201,13 @@ int rocbta_dpobtas(int64_t n, int64_t b, int64_t a, const double* D, const doubl
   if (trans != 'N' && trans != 'T' && trans != 'C') return ROCBTA_ERR_ARG;
   if (r == 0) return ROCBTA_OK;
   hipStream_t s = (hipStream_t)stream;
-  if (pobtas_use_sweep(partial)) {
+  const int algo = pobtas_algo(b, a, r, partial);
+  if (algo == SOLVE_BLOCK) {
+    if (!workspace || workspace_bytes < bs_layout(n, b, a, r).total) return ROCBTA_ERR_WORKSPACE;
+    return pobtas_block(n, b, a, D, L, R, T, B, r, ldb, trans == 'N' ? 0 : 1, partial,
+                        (char*)workspace, s);
+  }
+  if (algo == SOLVE_SWEEP) {
     if (!workspace || workspace_bytes < sw_layout(n, b, a, r).total) return ROCBTA_ERR_WORKSPACE;
     return pobtas_sweep(n, b, a, D, L, R, T, B, r, ldb, trans == 'N' ? 0 : 1, (char*)workspace, s);
   }
@@ -1285,10 +1477,16 @@ int rocbta_dquad(int64_t rows, int64_t r, const double* y, const double* x, int6
 }
 
 int rocbta_dlogdet(int64_t n, int64_t b, int64_t a, const double* D, const double* T,
-                   double* out, void* stream) {
+                   double* out, void* workspace, size_t workspace_bytes, void* stream) {
   if (bad_shape(n, b, a) || !D || !out || (a > 0 && !T)) return ROCBTA_ERR_ARG;
+  if (!workspace || workspace_bytes < rocbta_workspace_bytes(ROCBTA_OP_LOGDET, n, b, a, 0))
+    return ROCBTA_ERR_WORKSPACE;
   hipStream_t s = (hipStream_t)stream;
-  logdet_kernel<<<1, 256, 0, s>>>(D, n, b, T, a, out);
+  const int nparts = logdet_parts(n, b, a);
+  double* part = (double*)workspace;
+  logdet_part_kernel<<<nparts, 256, 0, s>>>(D, n, b, T, a, part);
+  RB_CHECK(hipGetLastError());
+  logdet_finish_kernel<<<1, 256, 0, s>>>(part, nparts, out);
   RB_CHECK(hipGetLastError());
   return ROCBTA_OK;
 }

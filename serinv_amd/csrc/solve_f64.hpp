@@ -86,11 +86,54 @@ __device__ __forceinline__ void logdet_body(const double* D, int64_t n, int64_t 
   if (threadIdx.x == 0) *out = 2.0 * red[0];
 }
 
+// Multi-workgroup form: workgroup w sums log diag over the rows
+// [w * rows_per_wg, (w + 1) * rows_per_wg) of the N = n * b + a diagonal
+// entries (block rows first, then the tip) into part[w]; logdet_finish_kernel
+// then adds the partials in index order, so the result does not depend on
+// scheduling.  Each thread handles consecutive rows: the diagonal entries
+// of one block are b + 1 doubles apart.
+constexpr int LD_ROWS_PER_WG = 1024;
+
+__device__ __forceinline__ double diag_entry(const double* D, int64_t n, int64_t b,
+                                             const double* T, int64_t a, int64_t row) {
+  if (row < n * b) {
+    const int64_t i = row / b, j = row - i * b;
+    return D[i * b * b + j * (b + 1)];
+  }
+  const int64_t j = row - n * b;
+  return T[j * (a + 1)];
+}
+
 __global__ void __launch_bounds__(256)
-    logdet_kernel(const double* D, int64_t n, int64_t b, const double* T, int64_t a,
-                  double* out) {
+    logdet_part_kernel(const double* D, int64_t n, int64_t b, const double* T, int64_t a,
+                       double* part) {
   __shared__ double red[256];
-  logdet_body(D, n, b, T, a, out, red);
+  const int64_t N = n * b + a;
+  const int64_t r0 = (int64_t)blockIdx.x * LD_ROWS_PER_WG;
+  const int64_t r1 = r0 + LD_ROWS_PER_WG < N ? r0 + LD_ROWS_PER_WG : N;
+  double s = 0.0;
+  for (int64_t row = r0 + threadIdx.x; row < r1; row += 256) s += log(diag_entry(D, n, b, T, a, row));
+  red[threadIdx.x] = s;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) part[blockIdx.x] = red[0];
+}
+
+__global__ void __launch_bounds__(256) logdet_finish_kernel(const double* part, int nparts,
+                                                            double* out) {
+  __shared__ double red[256];
+  double s = 0.0;
+  for (int k = threadIdx.x; k < nparts; k += 256) s += part[k];
+  red[threadIdx.x] = s;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) *out = 2.0 * red[0];
 }
 
 // One workgroup per matrix (batched plans).

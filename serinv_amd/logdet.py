@@ -9,7 +9,7 @@ pobtf.  Returns a 0-dim float64 tensor on the factor's device.
 import torch
 
 from . import _lib
-from .algs._common import _tensor, stream_of
+from .algs._common import _tensor, stream_of, workspace
 
 
 def pobta_logdet(L_diagonal_blocks, L_arrow_tip_block=None):
@@ -23,7 +23,8 @@ def pobta_logdet(L_diagonal_blocks, L_arrow_tip_block=None):
         _tensor(T, "arrow tip block")
         a = T.shape[0]
     out = torch.empty((), dtype=torch.float64, device=D.device)
+    ws, nbytes = workspace(_lib.OP_LOGDET, n, b, a, 0, D.device)
     rc = _lib.load().rocbta_dlogdet(n, b, a, D.data_ptr(), T.data_ptr() if a else None,
-                                    out.data_ptr(), stream_of(D.device))
+                                    out.data_ptr(), ws.data_ptr(), nbytes, stream_of(D.device))
     _lib.check(rc, "pobta_logdet")
     return out

@@ -26,17 +26,21 @@ def algs(gpu_device):
     return A
 
 
-@pytest.fixture(params=["dag", "loop"])
+@pytest.fixture(params=["dag", "loop", "block"])
 def pobtaf_algo(request, monkeypatch):
-    """Run the test with the persistent kernels (chain pobtaf, sweep pobtas)
-    and with the launch-per-tile forms (ROCBTA_POBTAF_ALGO=loop,
-    ROCBTA_POBTAS_ALGO=loop); the launch-per-tile run also selects the
-    128-wide GEMM tiles for the large products (ROCBTA_GEMM_TILE=128)."""
+    """Run the test with the persistent kernels (chain pobtaf; the solve
+    algorithm the shape selects), with the launch-per-tile forms
+    (ROCBTA_POBTAF_ALGO=loop, ROCBTA_POBTAS_ALGO=loop), and with the
+    block-level solve forced for every shape (ROCBTA_POBTAS_ALGO=block); the
+    launch-per-tile run also selects the 128-wide GEMM tiles for the large
+    products (ROCBTA_GEMM_TILE=128)."""
     for var in ("ROCBTA_POBTAF_ALGO", "ROCBTA_POBTAS_ALGO"):
         if request.param == "loop":
             monkeypatch.setenv(var, "loop")
         else:
             monkeypatch.delenv(var, raising=False)
+    if request.param == "block":
+        monkeypatch.setenv("ROCBTA_POBTAS_ALGO", "block")
     if request.param == "loop":
         monkeypatch.setenv("ROCBTA_GEMM_TILE", "128")
     else:
@@ -142,8 +146,13 @@ def test_bt_routines(algs, gpu_device, pobtaf_algo, n, b):
     assert relerr(to_host(d)[0], Do) < TOL and relerr(to_host(l)[0], Lo) < TOL
 
 
-@pytest.mark.parametrize("n,b,a", [(4, 3, 2), (3, 70, 5), (2, 64, 0)])
-def test_partial_flags(algs, gpu_device, n, b, a):
+@pytest.mark.parametrize("solve_algo", ["default", "block"])
+@pytest.mark.parametrize("n,b,a", [(4, 3, 2), (3, 70, 5), (2, 64, 0), (3, 256, 9)])
+def test_partial_flags(algs, gpu_device, monkeypatch, solve_algo, n, b, a):
+    if solve_algo == "block":
+        monkeypatch.setenv("ROCBTA_POBTAS_ALGO", "block")
+    else:
+        monkeypatch.delenv("ROCBTA_POBTAS_ALGO", raising=False)
     D, L, R, T, B = bta_spd(n, b, a, seed=900 + n, n_rhs=2)
     # factorize_last_block=False
     Do, Lo, Ro, To = (x.copy() for x in (D, L, R, T))
@@ -203,3 +212,19 @@ def test_errors(algs, gpu_device):
     with pytest.raises(np.linalg.LinAlgError) as ei:
         algs.pobtaf(d, l, r, t)
     assert "order 9 " in str(ei.value)  # first pivot of block 1 = global row 9
+
+
+@pytest.mark.parametrize("solve_algo", ["sweep", "block"])
+@pytest.mark.parametrize("r", [520, 1100])
+def test_wide_rhs(algs, gpu_device, monkeypatch, solve_algo, r):
+    monkeypatch.setenv("ROCBTA_POBTAS_ALGO", solve_algo)
+    """More right-hand-side columns than one sweep launch takes (64 chunks of
+    8 columns): the solve runs over several column groups."""
+    D, L, R, T, B = bta_spd(3, 40, 7, seed=31 + r, n_rhs=r)
+    Do, Lo, Ro, To, Bo = (x.copy() for x in (D, L, R, T, B))
+    serinv_np.pobtaf(Do, Lo, Ro, To)
+    d, l, rr, t, bb = to_dev(gpu_device, Do, Lo, Ro, To, B)
+    for trans in ("N", "C"):
+        serinv_np.pobtas(Do, Lo, Ro, To, Bo, trans=trans)
+        algs.pobtas(d, l, rr, t, bb, trans=trans)
+        assert relerr(to_host(bb)[0], Bo) < TOL, trans
