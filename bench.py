@@ -91,30 +91,39 @@ def _cpu_info():
 def cpu1_baseline(reps=5):
     """CPU-1: the oracle on n=32, b=64, a=16, r=1 (strictly diagonally
     dominant shifts), one warm-up, then the median of `reps` runs on fresh
-    copies; per-op milliseconds."""
+    copies; per-op milliseconds.  Timed with one BLAS thread and with all
+    cores; the faster setting is reported (64 x 64 blocks are too small for
+    threaded BLAS calls to pay off)."""
     import numpy as np
+    from threadpoolctl import threadpool_limits
 
     from oracle import serinv_np
     from serinv_amd.utils.matrices import bta_spd
 
     base = bta_spd(32, 64, 16, seed=63, n_rhs=1)
-    times = []
-    for k in range(reps + 1):
-        D, L, R, T, B = (x.copy() for x in base)
-        t0 = time.perf_counter()
-        serinv_np.pobtaf(D, L, R, T)
-        t1 = time.perf_counter()
-        serinv_np.pobtas(D, L, R, T, B, trans="N")
-        serinv_np.pobtas(D, L, R, T, B, trans="C")
-        t2 = time.perf_counter()
-        serinv_np.pobtasi(D, L, R, T)
-        t3 = time.perf_counter()
-        if k:
-            times.append((t1 - t0, t3 - t2, t2 - t1))
-    med = np.median(np.array(times), axis=0) * 1e3
+    best = None
+    for threads in (1, len(os.sched_getaffinity(0))):
+        times = []
+        with threadpool_limits(limits=threads):
+            for k in range(reps + 1):
+                D, L, R, T, B = (x.copy() for x in base)
+                t0 = time.perf_counter()
+                serinv_np.pobtaf(D, L, R, T)
+                t1 = time.perf_counter()
+                serinv_np.pobtas(D, L, R, T, B, trans="N")
+                serinv_np.pobtas(D, L, R, T, B, trans="C")
+                t2 = time.perf_counter()
+                serinv_np.pobtasi(D, L, R, T)
+                t3 = time.perf_counter()
+                if k:
+                    times.append((t1 - t0, t3 - t2, t2 - t1))
+        med = np.median(np.array(times), axis=0) * 1e3
+        if best is None or med[0] + med[1] < best[1][0] + best[1][1]:
+            best = (threads, med)
+    threads, med = best
     return {"workload": "n=32 b=64 a=16 r=1", "pobtaf_ms": round(float(med[0]), 3),
             "pobtasi_ms": round(float(med[1]), 3), "pobtas_NC_ms": round(float(med[2]), 3),
-            "runs": reps}
+            "runs": reps, "threads": threads}
 
 
 def cpu2_oracle(host, gpu, n, b, a):
@@ -159,7 +168,7 @@ def cpu2_oracle(host, gpu, n, b, a):
     return t_af, t_si, t_sv, threads, errs
 
 
-def bench_batched(dev, nmat=64, r=4, reps=3, tune=True):
+def bench_batched(dev, nmat=64, r=4, reps=3, tune=True, mixed=False):
     """64 independent SPD BTA systems (n ~ U{16..256}, b in {64, 128, 256},
     a ~ U{4..32}, seed 63 + k, 4 RHS): factor + solve + selected inversion,
     as one replayed batched plan (hipGraph) against a loop of the plain
@@ -190,6 +199,10 @@ def bench_batched(dev, nmat=64, r=4, reps=3, tune=True):
                 t.copy_(s)
         torch.cuda.synchronize(dev)
 
+    from serinv_amd.mixed import pobtaf_mixed, pobtasi_mixed
+
+    factor = pobtaf_mixed if mixed else algs.pobtaf
+    selinv = pobtasi_mixed if mixed else algs.pobtasi
     stream = torch.cuda.current_stream(dev)
     algs.config["check_info"] = False
     loop = []
@@ -198,10 +211,10 @@ def bench_batched(dev, nmat=64, r=4, reps=3, tune=True):
         e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
         e0.record(stream)
         for D, L, R, T, B in work:
-            algs.pobtaf(D, L, R, T)
+            factor(D, L, R, T)
             algs.pobtas(D, L, R, T, B, trans="N")
             algs.pobtas(D, L, R, T, B, trans="C")
-            algs.pobtasi(D, L, R, T)
+            selinv(D, L, R, T)
         e1.record(stream)
         torch.cuda.synchronize(dev)
         loop.append(e0.elapsed_time(e1))
@@ -210,8 +223,9 @@ def bench_batched(dev, nmat=64, r=4, reps=3, tune=True):
     tuned = None
     if tune:
         tuned = tune_batched([w[:4] for w in work], rhs=[w[4] for w in work], selinv=True,
-                             use_cache=False)
-    plan = BatchedBTA([w[:4] for w in work], rhs=[w[4] for w in work], selinv=True, graph=True)
+                             use_cache=False, mixed=mixed)
+    plan = BatchedBTA([w[:4] for w in work], rhs=[w[4] for w in work], selinv=True, graph=True,
+                      mixed=mixed)
     if tuned:
         plan.set_grid(tuned["factor_grid"], tuned["solve_grid"])
     restore()
@@ -238,7 +252,7 @@ def bench_batched(dev, nmat=64, r=4, reps=3, tune=True):
     t_loop, t_bat = min(loop), min(bat)
     return {
         "workload": f"{nmat} SPD BTA systems, n~U[16,256], b in {{64,128,256}}, a~U[4,32], r={r}; "
-                    "pobtaf + pobtas(N,C) + pobtasi, FP64",
+                    "pobtaf + pobtas(N,C) + pobtasi, " + ("mixed FP32/FP64" if mixed else "FP64"),
         "batched_ms": round(t_bat, 3),
         "loop_ms": round(t_loop, 3),
         "speedup": round(t_loop / t_bat, 2),
@@ -502,6 +516,9 @@ def main():
         out["batched_speedup"] = out["batched"]["speedup"]
         print(f"[bench] batched {out['batched']['batched_ms']} ms vs loop {out['batched']['loop_ms']} ms",
               file=sys.stderr, flush=True)
+        out["batched_mixed"] = bench_batched(dev, mixed=True)
+        print(f"[bench] batched mixed {out['batched_mixed']['batched_ms']} ms vs mixed loop "
+              f"{out['batched_mixed']['loop_ms']} ms", file=sys.stderr, flush=True)
     if not args.no_cpu_baseline:
         print("[bench] CPU-1 and CPU-2: the oracle on the host ...", file=sys.stderr, flush=True)
         model, blas, npver = _cpu_info()

@@ -143,9 +143,17 @@ int rocbta_dquad(int64_t rows, int64_t r, const double* y, const double* x, int6
 #define ROCBTA_BATCHED_FACTOR 1
 #define ROCBTA_BATCHED_SOLVE 2
 #define ROCBTA_BATCHED_SELINV 4
+/* with ROCBTA_BATCHED_MIXED the factorization and selected inversion run in
+ * the mixed form of rocbta_mpobtaf / rocbta_mpobtasi (FP32 tile products,
+ * FP64 tile potrf / trtri, tip and log-determinant); the solves stay FP64
+ * with that factor. */
+#define ROCBTA_BATCHED_MIXED 8
 typedef void* rocbta_batched_plan;
 
-/* Build a plan (allocates its own device workspace; not stream-ordered). */
+/* Build a plan (allocates its own device workspace; not stream-ordered).
+ * Systems run in waves of at most a quarter of the CU count (factorization)
+ * and 64 column chunks (solves), so any number of systems is accepted;
+ * r[m] <= 512 columns per system. */
 int rocbta_batched_create(int nmat, const int64_t* n, const int64_t* b, const int64_t* a,
                           double* const* D, double* const* L, double* const* R,
                           double* const* T, double* const* B, const int64_t* r, int ops,

@@ -16,7 +16,7 @@ ROCBTA_ERR_ARG = -1
 ROCBTA_ERR_HIP = -2
 ROCBTA_ERR_WORKSPACE = -3
 OP_POBTAF, OP_POBTASI, OP_POBTAS, OP_RESIDUAL, OP_LOGDET = 0, 1, 2, 3, 4
-BATCHED_FACTOR, BATCHED_SOLVE, BATCHED_SELINV = 1, 2, 4
+BATCHED_FACTOR, BATCHED_SOLVE, BATCHED_SELINV, BATCHED_MIXED = 1, 2, 4, 8
 
 _ERRORS = {
     ROCBTA_ERR_ARG: "invalid argument",

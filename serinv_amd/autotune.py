@@ -53,10 +53,11 @@ def _store(table):
         pass
 
 
-def tune_batched(systems, rhs=None, selinv=True, candidates=None, reps=2, use_cache=True):
+def tune_batched(systems, rhs=None, selinv=True, candidates=None, reps=2, use_cache=True,
+                 mixed=False):
     """Return {"factor_grid": g1, "solve_grid": g2, "ms": t, "trials": [...]}."""
     dev = systems[0][0].device
-    key = batch_class(systems, rhs) + (":si" if selinv else "")
+    key = batch_class(systems, rhs) + (":si" if selinv else "") + (":mx" if mixed else "")
     table = _load() if use_cache else {}
     if key in table:
         return table[key]
@@ -65,7 +66,7 @@ def tune_batched(systems, rhs=None, selinv=True, candidates=None, reps=2, use_ca
         candidates = [(f * cus, s * cus) for f in (1, 2, 3) for s in (1, 2)]
     scratch = [[t.clone() for t in sysm] for sysm in systems]
     rscratch = None if rhs is None else [x.clone() for x in rhs]
-    plan = BatchedBTA(scratch, rhs=rscratch, selinv=selinv, graph=False)
+    plan = BatchedBTA(scratch, rhs=rscratch, selinv=selinv, graph=False, mixed=mixed)
     trials = []
     try:
         for fg, sg in candidates:

@@ -12,6 +12,9 @@ log-determinants, overwrites each B with A^{-1} B (one persistent launch per
 sweep direction) and, with selinv=True, overwrites each factor with the
 selected inverse (merged grouped-GEMM phases).  With graph=True the launch
 sequence is captured into a hipGraph on the first run and replayed after.
+With mixed=True the factorization and the selected inversion run in the
+mixed FP32/FP64 form of serinv_amd.mixed (FP32 tile products; FP64 tile
+potrf/trtri, tip, log-determinant and solves).
 """
 
 import ctypes
@@ -24,7 +27,7 @@ from .algs._common import _tensor, bta_shapes
 
 
 class BatchedBTA:
-    def __init__(self, systems, rhs=None, selinv=True, graph=True):
+    def __init__(self, systems, rhs=None, selinv=True, graph=True, mixed=False):
         if not systems:
             raise ValueError("need at least one system")
         self._keep = []
@@ -72,6 +75,8 @@ class BatchedBTA:
             ops |= _lib.BATCHED_SOLVE
         if selinv:
             ops |= _lib.BATCHED_SELINV
+        if mixed:
+            ops |= _lib.BATCHED_MIXED
         I64 = ctypes.c_int64 * nmat
         P = ctypes.c_void_p * nmat
         self._arrays = [I64(*ns), I64(*bs), I64(*as_), I64(*(rs or [0] * nmat))]

@@ -229,6 +229,8 @@ int rocbta_batched_create(int nmat, const int64_t* n, const int64_t* b, const in
   }
   // one matrix's column chunks must fit one sweep wave
   if (solve && cdiv(rmax, sweep_rc(rmax)) > SW_MAX_CHAINS) return ROCBTA_ERR_ARG;
+  // chain_params and the recorded GEMM problems read the mixed flag
+  MixedScope mixed((ops & ROCBTA_BATCHED_MIXED) != 0);
   BatchedPlan* p = new BatchedPlan();
   p->nmat = nmat;
   p->ops = ops;

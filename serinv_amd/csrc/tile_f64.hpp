@@ -48,18 +48,18 @@ __device__ __forceinline__ double readlane_f64(double v, int lane) {
 }
 
 // d = sqrt(x), inv = 1/sqrt(x) for x > 0 (NaN propagates), from the hardware
-// reciprocal square root refined by Newton's iteration for 1/sqrt,
-//   y <- y + y (1 - x y^2) / 2,
-// twice (quadratic convergence from the ~2^-26 seed), then one Newton step
-// on d = x y for the square root itself, d <- d + y (x - d^2) / 2.  The
-// dependent chain is about a third of sqrt() followed by a division.
+// reciprocal square root y0 (relative error below 2^-22) and one third-order
+// correction of it: with e = 1 - x y0^2,
+//   1/sqrt(x) = y0 (1 - e)^(-1/2) = y0 (1 + e/2 + 3e^2/8 + O(e^3)),
+// so y = y0 + (y0 e)(1/2 + 3e/8) leaves a relative error of about 2.5 * 2^-66
+// before rounding (four dependent operations after the seed; two Newton
+// steps take six).  The square root is one Newton step on d = x y,
+// d <- d + y (x - d^2) / 2, which is not on the path of inv.
 __device__ __forceinline__ void pivot_sqrt(double x, double& d, double& inv) {
-  double y = __builtin_amdgcn_rsq(x);
-  const double h = 0.5 * x;
-  double e = __builtin_fma(-h * y, y, 0.5);
-  y = __builtin_fma(y, e, y);
-  e = __builtin_fma(-h * y, y, 0.5);
-  y = __builtin_fma(y, e, y);
+  const double y0 = __builtin_amdgcn_rsq(x);
+  const double e = __builtin_fma(-(x * y0), y0, 1.0);
+  const double p = __builtin_fma(0.375, e, 0.5);
+  const double y = __builtin_fma(y0 * e, p, y0);
   double s = x * y;
   const double rr = __builtin_fma(-s, s, x);
   s = __builtin_fma(0.5 * y, rr, s);
@@ -89,14 +89,17 @@ __device__ __forceinline__ void wave_panel_potrf(double (*S)[TT_LD], int p, int 
       }
       double d, inv;
       pivot_sqrt(piv, d, inv);
-      r[j] = (lane == j) ? d : (lane > j ? r[j] * inv : r[j]);
+      // the column is broadcast from its scaled value; the diagonal entry d
+      // (lane j) is not on the path to the next pivot
+      const double sc = r[j] * inv;
 #pragma unroll
       for (int k = j + 1; k < PW; ++k) {
         if (k < w) {
-          const double lkj = readlane_f64(r[j], k);
-          r[k] -= r[j] * lkj;
+          const double lkj = readlane_f64(sc, k);
+          r[k] -= sc * lkj;
         }
       }
+      r[j] = (lane == j) ? d : (lane > j ? sc : r[j]);
     }
   }
 #pragma unroll
@@ -137,9 +140,12 @@ __device__ __forceinline__ void wave_panel_potrf16(double (*S)[TT_LD], int p, in
       bad = (bad == PW && !(piv > 0.0)) ? j : bad;
       double d, inv;
       pivot_sqrt(piv, d, inv);
-      r[j] = (lane == j) ? d : r[j] * inv;
+      // broadcasts read the scaled column before the diagonal entry d is
+      // selected in (d is off the path to the next pivot)
+      const double sc = r[j] * inv;
 #pragma unroll
-      for (int k = j + 1; k < c0 + 4; ++k) r[k] -= r[j] * readlane_f64(r[j], k);
+      for (int k = j + 1; k < c0 + 4; ++k) r[k] -= sc * readlane_f64(sc, k);
+      r[j] = (lane == j) ? d : sc;
     }
 #pragma unroll
     for (int q = 0; q < 4; ++q)

@@ -119,3 +119,48 @@ def test_batched_more_systems_than_cus(gpu_device):
         assert relerr(to_host(d[4])[0], x) < TOL, f"system {k} solve"
         for nm, g, o in zip("DLRT", to_host(*d[:4]), X):
             assert relerr(g, o) < TOL, f"system {k} X_{nm}: {relerr(g, o):.2e}"
+
+
+def test_batched_mixed(gpu_device):
+    """Mixed FP32/FP64 plan: every system against the FP64 oracle at the bar
+    of the mixed form (FP32 products, 1e-5), and its factor equal to the
+    per-matrix mixed factorization (same tile kernels)."""
+    from serinv_amd.batched import BatchedBTA
+    from serinv_amd.logdet import pobta_logdet
+    from serinv_amd.mixed import pobtaf_mixed
+
+    host, dev = _make(gpu_device, SHAPES, seed0=1500)
+    single = []
+    for d in dev:
+        f = [t.clone() for t in d[:4]]
+        pobtaf_mixed(*f)
+        single.append((f, float(pobta_logdet(f[0], f[3]))))
+    plan = BatchedBTA([d[:4] for d in dev], rhs=[d[4] for d in dev], selinv=False, graph=True,
+                      mixed=True)
+    plan.run()
+    info, logdet, status = plan.results()
+    plan.close()
+    assert status == 0 and np.all(info == 0), (info, status)
+    for k, (h, d) in enumerate(zip(host, dev)):
+        D, L, R, T, B = (x.copy() for x in h)
+        serinv_np.pobtaf(D, L, R, T)
+        ld = serinv_np.logdet_from_factor(D, T)
+        for nm, g, s, o in zip("DLRT", to_host(*d[:4]), to_host(*single[k][0]), (D, L, R, T)):
+            assert np.array_equal(g, s), f"system {k} factor {nm} differs from pobtaf_mixed"
+            assert relerr(g, o) < 1e-5, f"system {k} factor {nm}: {relerr(g, o):.2e}"
+        assert logdet[k] == single[k][1] or abs(logdet[k] - single[k][1]) <= 1e-14 * abs(ld)
+        assert abs(logdet[k] - ld) <= 1e-7 * abs(ld)
+        serinv_np.pobtas(D, L, R, T, B, trans="N")
+        serinv_np.pobtas(D, L, R, T, B, trans="C")
+        assert relerr(to_host(d[4])[0], B) < 1e-5, f"system {k} solve"
+    # selected inversion in the mixed plan
+    host, dev = _make(gpu_device, SHAPES[:4], seed0=1600)
+    plan = BatchedBTA([d[:4] for d in dev], selinv=True, graph=False, mixed=True)
+    plan.run()
+    info, _, _ = plan.results()
+    plan.close()
+    assert np.all(info == 0)
+    for k, (h, d) in enumerate(zip(host, dev)):
+        _, _, X = _oracle(*h)
+        for nm, g, o in zip("DLRT", to_host(*d[:4]), X):
+            assert relerr(g, o) < 1e-5, f"system {k} X_{nm}: {relerr(g, o):.2e}"
