@@ -8,6 +8,7 @@ import os
 import shutil
 import subprocess
 import sys
+from concurrent.futures import ThreadPoolExecutor
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 PKG = os.path.dirname(HERE)
@@ -19,7 +20,6 @@ HIPCC_FLAGS = [
     "--offload-arch=gfx950",
     "-O3",
     "-std=c++17",
-    "-shared",
     "-fPIC",
     # keep the compiler from emitting packed FP32 math (v_pk_*_f32)
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
@@ -33,10 +33,7 @@ def hipcc_path():
     raise RuntimeError("hipcc not found: cannot build librocbta.so")
 
 
-def build(verbose=False):
-    os.makedirs(LIB_DIR, exist_ok=True)
-    tmp = LIB_PATH + ".tmp"
-    cmd = [hipcc_path(), *HIPCC_FLAGS, "-o", tmp] + [os.path.join(HERE, s) for s in SOURCES]
+def _run(cmd, verbose):
     if verbose:
         print(" ".join(cmd), flush=True)
     proc = subprocess.run(cmd, capture_output=True, text=True)
@@ -47,7 +44,24 @@ def build(verbose=False):
         raise RuntimeError(f"hipcc failed ({proc.returncode}):\n{msgs}")
     if verbose and msgs.strip():
         print(msgs, flush=True)
+
+
+def build(verbose=False):
+    """Compile every source to an object (in parallel), then link the library."""
+    os.makedirs(LIB_DIR, exist_ok=True)
+    hipcc = hipcc_path()
+    objs = [os.path.join(LIB_DIR, os.path.splitext(s)[0] + ".o") for s in SOURCES]
+    jobs = [[hipcc, *HIPCC_FLAGS, "-c", "-o", o, os.path.join(HERE, s)]
+            for s, o in zip(SOURCES, objs)]
+    workers = max(1, min(len(jobs), int(os.environ.get("MAX_JOBS", os.cpu_count() or 1))))
+    with ThreadPoolExecutor(max_workers=workers) as pool:
+        for f in [pool.submit(_run, cmd, verbose) for cmd in jobs]:
+            f.result()
+    tmp = LIB_PATH + ".tmp"
+    _run([hipcc, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp, *objs], verbose)
     os.replace(tmp, LIB_PATH)
+    for o in objs:
+        os.remove(o)
     return LIB_PATH
 
 
