@@ -56,6 +56,21 @@ int rocbta_dpobtaf(int64_t n, int64_t b, int64_t a, double* D, double* L, double
                    int factorize_last_block, int* info, void* workspace, size_t workspace_bytes,
                    void* stream);
 
+/* pobtaf without any device allocation.  No counterpart in the reference:
+ * for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  Same
+ * arguments, kernels and results as rocbta_dpobtaf, but the workspace is a
+ * static device array of ROCBTA_NOALLOC_SCRATCH_BYTES per GPU, held until
+ * the process ends (the limit: rocbta_workspace_bytes(ROCBTA_OP_POBTAF, n,
+ * b, a, 0) must not exceed it, about 32 KiB per 64 x 64 diagonal tile; e.g.
+ * n = 256, b = 256, a = 32 fits; larger shapes return ROCBTA_ERR_WORKSPACE
+ * with a message on stderr).  The array is shared by every call on one GPU: calls that overlap
+ * on one GPU (two streams, two host threads) race. */
+#define ROCBTA_NOALLOC_SCRATCH_BYTES ((size_t)40 << 20)
+size_t rocbta_noalloc_scratch_bytes(void);
+int rocbta_dpobtaf_noalloc(int64_t n, int64_t b, int64_t a, double* D, double* L, double* R,
+                           double* T, int factorize_last_block, int* info, void* stream);
+
 /* Selected inversion from the block Cholesky factor, in place.
  * Replaces serinv.algs.pobtasi / _pobtasi (src/serinv/algs/pobtasi.py:10-89,
  * 92-172) and, with a == 0, serinv.algs.pobtsi / _pobtsi (pobtsi.py:10-125).

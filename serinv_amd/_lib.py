@@ -35,6 +35,9 @@ SIGNATURES = {
     "rocbta_workspace_bytes": (_c_size, [_c_int, _c_i64, _c_i64, _c_i64, _c_i64]),
     "rocbta_dpobtaf": (_c_int, [_c_i64, _c_i64, _c_i64, _c_p, _c_p, _c_p, _c_p, _c_int, _c_p,
                                 _c_p, _c_size, _c_p]),
+    "rocbta_noalloc_scratch_bytes": (_c_size, []),
+    "rocbta_dpobtaf_noalloc": (_c_int, [_c_i64, _c_i64, _c_i64, _c_p, _c_p, _c_p, _c_p, _c_int, _c_p,
+                                        _c_p]),
     "rocbta_dpobtasi": (_c_int, [_c_i64, _c_i64, _c_i64, _c_p, _c_p, _c_p, _c_p, _c_int, _c_p,
                                  _c_size, _c_p]),
     "rocbta_dpobtas": (_c_int, [_c_i64, _c_i64, _c_i64, _c_p, _c_p, _c_p, _c_p, _c_p, _c_i64,
