@@ -61,7 +61,24 @@ struct PobtafChain {
   // FP64 operands to FP32 and accumulate in FP32 MFMA; the tiles stay FP64
   // in memory and the diagonal potrf / trtri and the tip run in FP64.
   int f32;
+  // Chain schedule for full 64-wide tiles (b % 64 == 0): the sub-diagonal
+  // solve is formed by column blocks around the tile factorization
+  // (chain_run_fused); 0 selects the step-after-step schedule (chain_run).
+  int fused;
+  // Diagnostic timeline (nullptr normally): 100 MHz timestamps, CHAIN_TRACE_EV
+  // per chain step g at trace[CHAIN_TRACE_EV * g + e], see chain_trace.
+  long long* trace;
 };
+
+// Timeline events of the fused chain step g (diagnostic builds of a run):
+//   0 step start, 1 factorization done, 2 PX(g)/PD(g+1) staged,
+//   3 column blocks 0-2 and Linv(g) done, 4 L(g+1,g) complete, 5 step end,
+//   6 PX(g) published (by its helper), 7 L(g+2,g) published (by its helper).
+constexpr int CHAIN_TRACE_EV = 8;
+__device__ __forceinline__ void chain_trace(const PobtafChain& P, int g, int e) {
+  if (P.trace && threadIdx.x == 0)
+    P.trace[CHAIN_TRACE_EV * g + e] = (long long)__builtin_amdgcn_s_memrealtime();
+}
 
 // Hardware key of the CU running the calling wave: XCC id and the
 // SE/SH/CU fields of HW_ID.
@@ -267,10 +284,10 @@ __device__ __forceinline__ void wave_stores_done() { __builtin_amdgcn_s_waitcnt(
 // The explicit wait after thread 0's release fence keeps the flag store
 // behind the completion of the L2 write-back: with the counters already
 // drained, the compiler otherwise emits no wait between the two.
-__device__ __forceinline__ void chain_publish(int* f) {
+__device__ __forceinline__ void chain_publish(int* f, int publisher = 0) {
   wave_stores_done();
   __syncthreads();
-  if (threadIdx.x == 0) {
+  if ((int)threadIdx.x == publisher) {
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
     wave_stores_done();
     __hip_atomic_store(f, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -560,6 +577,14 @@ __device__ __forceinline__ void chain_run(const PobtafChain& P, double* S, doubl
   f64x4 pd_next[2][2];
   bool have_pd = false;
   __shared__ int s_early;
+  // L(g+1,g) is stored at the end of step g but published during step g+1,
+  // after its syrk products have read the tile from LDS: the publish
+  // (every wave's store wait, a barrier, the release) then replaces the
+  // barrier that follows the products instead of adding one, and the
+  // release runs on wave 3, which has no work in the first panel of the
+  // tile factorization that follows.
+  int pending = -1;  // step whose L(g+1,g) stores are not yet published
+  constexpr int PUBLISHER = 3 * 64;
   for (int g = 0; g < P.NT; ++g) {
     const CTile dt = tile_g(P, g, g);
     f64x4 acc[2][2];
@@ -589,6 +614,8 @@ __device__ __forceinline__ void chain_run(const PobtafChain& P, double* S, doubl
           a32[1][1] = mfma_acc(a1, b1, a32[1][1]);
         }
       }
+      if (pending >= 0) chain_publish(P.cflags + 2 * pending + 1, PUBLISHER);
+      pending = -1;
       acc32_add(a32, acc, S, TT_LD);
     } else if (g > 0) {
       // acc -= L(g,g-1) L(g,g-1)^T, with L(g,g-1) in S as S[row][k]
@@ -605,7 +632,11 @@ __device__ __forceinline__ void chain_run(const PobtafChain& P, double* S, doubl
           acc[1][1] = mfma_f64(a1, b1, acc[1][1]);
         }
       }
-      __syncthreads();
+      if (pending >= 0)
+        chain_publish(P.cflags + 2 * pending + 1, PUBLISHER);  // includes the barrier
+      else
+        __syncthreads();
+      pending = -1;
     }
     cacc_to_lds(acc, S, TT_LD);
     __syncthreads();
@@ -780,7 +811,316 @@ __device__ __forceinline__ void chain_run(const PobtafChain& P, double* S, doubl
     cacc_to_lds(acc, S, TT_LD);  // keep L(g+1,g) as S[row][k]
     __syncthreads();
     lds_tile_store(S, TT_LD, xt);
-    chain_publish(P.cflags + 2 * g + 1);
+    pending = g;
+  }
+  if (pending >= 0) chain_publish(P.cflags + 2 * pending + 1, PUBLISHER);
+}
+
+
+// Column block J of L(g+1,g) = PX(g) Linv(g)^T (rows 0..63, K < 16J + 16),
+// with a compile-time trip count so that the LDS operand loads are
+// scheduled ahead of the products.  ACC f64x4: FP64 MFMA; f32x4: FP32.
+template <int J, typename ACC>
+__device__ __forceinline__ void chain_colblock(const double* Ps, const double (*X2)[TT_LD],
+                                               ACC (&cb)[4]) {
+  const int lane = threadIdx.x & 63, li = lane & 15, lk = lane >> 4;
+#pragma unroll
+  for (int rb = 0; rb < 4; ++rb) cb[rb] = ACC{0, 0, 0, 0};
+#pragma unroll
+  for (int kk = 0; kk < 16 * J + 16; kk += 4) {
+    const double bv = X2[16 * J + li][kk + lk];
+#pragma unroll
+    for (int rb = 0; rb < 4; ++rb) cb[rb] = mfma_acc(Ps[(16 * rb + li) * TT_LD + kk + lk], bv, cb[rb]);
+  }
+}
+
+// ------------------------------------------------------- fused chain step
+// Chain schedule for matrices whose diagonal tiles are all full (b % 64 ==
+// 0).  Step g starts with D(g,g) fully updated in S and:
+//   1. factors it (wave 0 factors the 16-column panels; waves 1-3 update the
+//      trailing columns and form block rows 0-2 of Linv(g) beside them);
+//   2. with PX(g) staged in Ps, waves 1-3 form column blocks 0-2 of
+//      L(g+1,g) = PX(g) Linv(g)^T (column block J needs only block row J of
+//      Linv, already formed), while wave 0 forms X_33 = L(g,g)[48:64,48:64]^{-1};
+//   3. forms column block 3 by substitution (every wave 16 rows):
+//        L(g+1,g)[:, 48:64] = (PX(g)[:, 48:64] - L(g+1,g)[:, 0:48] L(g,g)[48:64, 0:48]^T) X_33^T,
+//      and waves 1-3 then form X_30, X_31, X_32, the rest of Linv's block row 3;
+//   4. D(g+1,g+1) = PD(g+1) - L(g+1,g) L(g+1,g)^T on waves 0, 2, 3 (the lower
+//      quadrants) into the other D buffer.
+// Wave 1 stores Linv(g) after step 3 and releases it during step 4, where
+// it has no quadrant; L(g+1,g) is released at the start of step g+1, whose
+// first panel only wave 0 works on.  In the mixed mode the products
+// (column blocks, the column-block-3 update and its X_33 product, the syrk)
+// round their operands to FP32 and accumulate in FP32; the differences and
+// the PD(g+1) term stay FP64.
+// LDS: two D buffers [64][65] (D(g,g) -> L(g,g), and D(g+1,g+1)), Xs
+// [64][65] (Linv(g)), Ws [64][17] (scratch), Ps [64][65] (PX(g) -> L(g+1,g),
+// row-major = K-contiguous).
+template <int TAG>
+__device__ __forceinline__ void chain_run_fused(const PobtafChain& P, double* Sa, double* Sb,
+                                                double* Xs, double* Ws, double* Ps) {
+  const int tid = threadIdx.x;
+  double (*X2)[TT_LD] = reinterpret_cast<double (*)[TT_LD]>(Xs);
+  const int lane = tid & 63, wave = tid >> 6;
+  const int wm = (wave >> 1) * 32, wn = (wave & 1) * 32;
+  const int li = lane & 15, lk = lane >> 4;
+  const int rs = tid >> 5, c2 = (tid & 31) * 2;
+  constexpr int PUB = 64;  // wave 1, lane 0: runs the releases
+  for (int idx = tid; idx < CT * TT_LD; idx += CH_THREADS) Xs[idx] = 0.0;
+  {
+    f64x4 acc[2][2];
+    cacc_load(tile_g(P, 0, 0), acc);  // D(0,0) has no updates
+    cacc_to_lds(acc, Sa, TT_LD);
+  }
+  __syncthreads();
+  f64x4 pd_next[2][2];
+  __shared__ int s_early;
+  int pending = -1;  // step whose Linv and L(g+1,g) stores wait for their release
+  double* S = Sa;
+  double* Sn = Sb;
+  for (int g = 0; g < P.NT; ++g) {
+    double (*S2)[TT_LD] = reinterpret_cast<double (*)[TT_LD]>(S);
+    const CTile dt = tile_g(P, g, g);
+    const bool sub = g + 1 < band_end(P, g);
+    const int* f_px = P.hflags + t_px(P, g);
+    const int* f_pd = sub ? P.hflags + t_pd(P, g + 1) : f_px;
+    if (pending >= 0 && tid == PUB) {
+      // every wave waited for its L(g,g-1) stores before the barrier that
+      // ended the previous step
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+      wave_stores_done();
+      __hip_atomic_store(P.cflags + 2 * pending + 1, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    pending = -1;
+    chain_trace(P, g, 0);
+    // 1. factorization with Linv block rows 0-2 beside it; PX(g) and
+    //    PD(g+1) flags checked by wave 1 during the last panel
+    const int64_t row0 = (int64_t)(g / P.nt) * P.b + (int64_t)(g % P.nt) * CT;
+    lds_potrf_t<CT>(S2, CT, P.info, row0, [&](int p) {
+      if (p == 0 && wave == 3) wave_diag_inv16(S2, X2, 0);
+      if (p == 16 && wave == 2) {
+        wave_diag_inv16(S2, X2, 16);
+        __builtin_amdgcn_wave_barrier();
+        wave_trtri_offdiag(S2, X2, &X2[16][16], TT_LD, 1, 0);
+      }
+      if (p == 32) {
+        if (wave == 1) {
+          int ok = sub ? (__hip_atomic_load(f_px, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) &
+                          __hip_atomic_load(f_pd, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
+                       : 0;
+          ok = __builtin_amdgcn_readfirstlane(ok);
+          if (ok) __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+          if (lane == 0) s_early = ok;
+        } else if (wave == 2) {
+          wave_diag_inv16(S2, X2, 32);
+          __builtin_amdgcn_wave_barrier();
+          wave_trtri_offdiag(S2, X2, &X2[32][32], TT_LD, 2, 1);
+        } else {
+          wave_diag_inv16_to(S2, 32, Ws, WS_LD);
+          __builtin_amdgcn_wave_barrier();
+          wave_trtri_offdiag(S2, X2, Ws, WS_LD, 2, 0);
+        }
+      }
+    });
+    const int early = s_early;
+    chain_trace(P, g, 1);
+    // L(g,g) to memory, upper triangle zero (16-byte stores)
+    {
+      cdbl2 v[8];
+#pragma unroll
+      for (int q = 0; q < 8; ++q) {
+        const int rr = rs + 8 * q;
+        v[q].x = (c2 <= rr) ? S2[rr][c2] : 0.0;
+        v[q].y = (c2 + 1 <= rr) ? S2[rr][c2 + 1] : 0.0;
+      }
+#pragma unroll
+      for (int q = 0; q < 8; ++q)
+        *reinterpret_cast<cdbl2*>(dt.p + (int64_t)(rs + 8 * q) * dt.ld + c2) = v[q];
+    }
+    double* li_out = P.linv + (int64_t)g * CT * CT;
+    if (!sub) {
+      // last diagonal tile of the matrix: Linv block row 3 and its release
+      if (wave < 3) {
+        double* xi = (wave == 0) ? &X2[48][48] : Ws + 16 * wave * WS_LD;
+        const int ldi = (wave == 0) ? TT_LD : WS_LD;
+        wave_diag_inv16_to(S2, 48, xi, ldi);
+        __builtin_amdgcn_wave_barrier();
+        wave_trtri_offdiag(S2, X2, xi, ldi, 3, 2 - wave);
+      }
+      __syncthreads();
+      cdbl2 v[8];
+#pragma unroll
+      for (int q = 0; q < 8; ++q) {
+        v[q].x = X2[rs + 8 * q][c2];
+        v[q].y = X2[rs + 8 * q][c2 + 1];
+      }
+#pragma unroll
+      for (int q = 0; q < 8; ++q) *reinterpret_cast<cdbl2*>(li_out + (rs + 8 * q) * CT + c2) = v[q];
+      chain_publish(P.cflags + 2 * g);
+      continue;
+    }
+    // PX(g) and PD(g+1): loaded now if both were published during the
+    // factorization, else each wave waits for them itself
+    const CTile xt = tile_g(P, g + 1, g);
+    TileRegs16 rx;
+    if (!early) wave_wait(P, f_px, f_pd);
+    tile_load16(xt, rx);
+    cacc_load(tile_g(P, g + 1, g + 1), pd_next);
+    // Ps[m][k] = PX(g)(m, k) (pitch 65: 8-byte stores)
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+      Ps[(rs + 8 * q) * TT_LD + c2] = rx.v[q].x;
+      Ps[(rs + 8 * q) * TT_LD + c2 + 1] = rx.v[q].y;
+    }
+    __syncthreads();
+    chain_trace(P, g, 2);
+    // 2. wave 0: block row 3 of Linv (X_33, then X_32, X_31, X_30);
+    //    wave J+1: column block J of L(g+1,g) = sum over k < 16J + 16 of
+    //    PX(:, k) Linv(16J + n, k)
+    f64x4 cb[4];
+    f32x4 cb32[4];
+    if (wave == 0) {
+      wave_diag_inv16(S2, X2, 48);
+      __builtin_amdgcn_wave_barrier();
+#pragma unroll
+      for (int J = 2; J >= 0; --J) wave_trtri_offdiag(S2, X2, &X2[48][48], TT_LD, 3, J);
+    } else if (!P.f32) {
+      if (wave == 1) chain_colblock<0>(Ps, X2, cb);
+      else if (wave == 2) chain_colblock<1>(Ps, X2, cb);
+      else chain_colblock<2>(Ps, X2, cb);
+    } else {
+      if (wave == 1) chain_colblock<0>(Ps, X2, cb32);
+      else if (wave == 2) chain_colblock<1>(Ps, X2, cb32);
+      else chain_colblock<2>(Ps, X2, cb32);
+    }
+    __syncthreads();  // every read of PX(g)'s column blocks 0-2 is done; Linv(g) is complete
+    if (wave == 1) {
+      // Linv(g) to memory by wave 1 alone (released by it in step 4), two
+      // whole rows per store instruction
+      const int hr = lane >> 5, cc = (lane & 31) * 2;
+#pragma unroll 8
+      for (int k = 0; k < 32; ++k) {
+        const int row = 2 * k + hr;
+        cdbl2 v;
+        v.x = X2[row][cc];
+        v.y = X2[row][cc + 1];
+        *reinterpret_cast<cdbl2*>(li_out + row * CT + cc) = v;
+      }
+    }
+    if (wave >= 1) {
+      const int J = wave - 1;
+#pragma unroll
+      for (int rb = 0; rb < 4; ++rb)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          if (!P.f32)
+            Ps[(16 * rb + acc_row_f64(lane, r)) * TT_LD + 16 * J + acc_col(lane)] = cb[rb][r];
+          else
+            Ps[(16 * rb + acc_row_f32(lane, r)) * TT_LD + 16 * J + acc_col(lane)] = (double)cb32[rb][r];
+        }
+    }
+    __syncthreads();  // column blocks 0-2 of L(g+1,g) in Ps
+    chain_trace(P, g, 3);
+    // 3. column block 3, wave w: rows 16w .. 16w + 15.  Ws receives the
+    //    product L(g+1,g)[:, 0:48] L(g,g)[48:64, 0:48]^T; the operand of the
+    //    X_33 product is PX(g)[:, 48:64] minus it, formed in FP64.
+    {
+      const int r0 = 16 * wave;
+      if (!P.f32) {
+        f64x4 v = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+        for (int kk = 0; kk < 48; kk += 4)
+          v = mfma_f64(Ps[(r0 + li) * TT_LD + kk + lk], S[(48 + li) * TT_LD + kk + lk], v);
+#pragma unroll
+        for (int r = 0; r < 4; ++r) Ws[(r0 + acc_row_f64(lane, r)) * WS_LD + acc_col(lane)] = v[r];
+      } else {
+        f32x4 v = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int kk = 0; kk < 48; kk += 4)
+          v = mfma_acc(Ps[(r0 + li) * TT_LD + kk + lk], S[(48 + li) * TT_LD + kk + lk], v);
+#pragma unroll
+        for (int r = 0; r < 4; ++r)
+          Ws[(r0 + acc_row_f32(lane, r)) * WS_LD + acc_col(lane)] = (double)v[r];
+      }
+      __builtin_amdgcn_wave_barrier();
+      if (!P.f32) {
+        f64x4 y = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+        for (int kk = 0; kk < 16; kk += 4) {
+          const double av = Ps[(r0 + li) * TT_LD + 48 + kk + lk] - Ws[(r0 + li) * WS_LD + kk + lk];
+          y = mfma_f64(av, X2[48 + li][48 + kk + lk], y);
+        }
+#pragma unroll
+        for (int r = 0; r < 4; ++r) Ps[(r0 + acc_row_f64(lane, r)) * TT_LD + 48 + acc_col(lane)] = y[r];
+      } else {
+        f32x4 y = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int kk = 0; kk < 16; kk += 4) {
+          const double av = Ps[(r0 + li) * TT_LD + 48 + kk + lk] - Ws[(r0 + li) * WS_LD + kk + lk];
+          y = mfma_acc(av, X2[48 + li][48 + kk + lk], y);
+        }
+#pragma unroll
+        for (int r = 0; r < 4; ++r)
+          Ps[(r0 + acc_row_f32(lane, r)) * TT_LD + 48 + acc_col(lane)] = (double)y[r];
+      }
+    }
+    __syncthreads();  // L(g+1,g) complete in Ps
+    chain_trace(P, g, 4);
+    lds_tile_store(Ps, TT_LD, xt);
+    pending = g;
+    // 4. D(g+1,g+1) = PD(g+1) - L(g+1,g) L(g+1,g)^T on the lower quadrants
+    //    (waves 0, 2, 3) into Sn; wave 1, which has no lower quadrant, waits
+    //    for its Linv(g) stores and releases them
+    f32x4 s32[2][2];
+    cacc32_zero(s32);
+    if (wn <= wm) {
+      if (!P.f32) {
+#pragma unroll
+        for (int kk = 0; kk < CT; kk += 4) {
+          const double a0 = -Ps[(wm + li) * TT_LD + kk + lk];
+          const double a1 = -Ps[(wm + 16 + li) * TT_LD + kk + lk];
+          const double b0 = Ps[(wn + li) * TT_LD + kk + lk];
+          const double b1 = Ps[(wn + 16 + li) * TT_LD + kk + lk];
+          pd_next[0][0] = mfma_f64(a0, b0, pd_next[0][0]);
+          pd_next[0][1] = mfma_f64(a0, b1, pd_next[0][1]);
+          pd_next[1][0] = mfma_f64(a1, b0, pd_next[1][0]);
+          pd_next[1][1] = mfma_f64(a1, b1, pd_next[1][1]);
+        }
+      } else {
+#pragma unroll
+        for (int kk = 0; kk < CT; kk += 4) {
+          const double a0 = -Ps[(wm + li) * TT_LD + kk + lk];
+          const double a1 = -Ps[(wm + 16 + li) * TT_LD + kk + lk];
+          const double b0 = Ps[(wn + li) * TT_LD + kk + lk];
+          const double b1 = Ps[(wn + 16 + li) * TT_LD + kk + lk];
+          s32[0][0] = mfma_acc(a0, b0, s32[0][0]);
+          s32[0][1] = mfma_acc(a0, b1, s32[0][1]);
+          s32[1][0] = mfma_acc(a1, b0, s32[1][0]);
+          s32[1][1] = mfma_acc(a1, b1, s32[1][1]);
+        }
+      }
+    } else {
+      wave_stores_done();
+      if (lane == 0) {
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+        wave_stores_done();
+        __hip_atomic_store(P.cflags + 2 * g, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      }
+    }
+    // FP32 products join the FP64 PD(g+1) through Sn (free until below)
+    if (P.f32) acc32_add(s32, pd_next, Sn, TT_LD);
+    cacc_to_lds(pd_next, Sn, TT_LD);
+    wave_stores_done();  // Linv(g), L(g,g), L(g+1,g): released at the start of step g+1
+    __syncthreads();
+    chain_trace(P, g, 5);
+    double* t = S;
+    S = Sn;
+    Sn = t;
+  }
+  if (pending >= 0 && tid == PUB) {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+    wave_stores_done();
+    __hip_atomic_store(P.cflags + 2 * pending + 1, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   }
 }
 
@@ -863,6 +1203,8 @@ __device__ __forceinline__ void chain_helper_task(const PobtafChain& P, int task
   __syncthreads();
   lds_tile_store(XA, KP, out);
   chain_publish(P.hflags + task);
+  if (kind == HK_PX) chain_trace(P, j, 6);
+  if (kind == HK_TRSM && r == j + 2) chain_trace(P, j, 7);
   // The strict-upper mirror of a lower D-block tile is zero in the factor.
   // No task reads it, so it is cleared after the publish, off the path of
   // the tasks waiting for this one.
@@ -870,8 +1212,14 @@ __device__ __forceinline__ void chain_helper_task(const PobtafChain& P, int task
   if (kind == HK_PX && (j + 1) / P.nt == j / P.nt) tile_zero(tile_mirror(P, j + 1, j));
 }
 
+// LDS of the single-matrix kernel: the helper plan (XA, XB) or the chain's
+// (S, Xs, Ws, and Ps for the fused schedule), whichever is larger.
+constexpr int CH_LDS_HELPER = 2 * CT * KP + CT * WS_LD;
+constexpr int CH_LDS_FUSED = 4 * CT * TT_LD + CT * WS_LD;
+constexpr int CH_LDS = CH_LDS_HELPER > CH_LDS_FUSED ? CH_LDS_HELPER : CH_LDS_FUSED;
+
 __global__ void __launch_bounds__(CH_THREADS) pobtaf_chain_kernel(PobtafChain P) {
-  __shared__ __attribute__((aligned(16))) double lds[2 * CT * KP + CT * WS_LD];
+  __shared__ __attribute__((aligned(16))) double lds[CH_LDS];
   __shared__ int s_task;
   const int tid = threadIdx.x;
   if (tid == 0) s_task = atomicAdd(P.claim, 1);
@@ -882,7 +1230,11 @@ __global__ void __launch_bounds__(CH_THREADS) pobtaf_chain_kernel(PobtafChain P)
     // the chain runs latency-bound code: keep helpers off its CU
     if (tid == 0)
       __hip_atomic_store(P.chain_cu, 1 + hw_cu_key(), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    chain_run<0>(P, lds, lds + CT * TT_LD, lds + 2 * CT * TT_LD);
+    if (P.fused)
+      chain_run_fused<0>(P, lds, lds + CT * TT_LD, lds + 2 * CT * TT_LD, lds + 3 * CT * TT_LD,
+                         lds + 3 * CT * TT_LD + CT * WS_LD);
+    else
+      chain_run<0>(P, lds, lds + CT * TT_LD, lds + 2 * CT * TT_LD);
     return;
   }
   const int my_cu = 1 + hw_cu_key();

@@ -578,7 +578,30 @@ PobtafChain chain_params(int64_t n, int64_t b, int64_t a, double* D, double* L, 
   P.info = info;
   P.spin_limit = 1ll << 22;
   P.f32 = g_mixed ? 1 : 0;
+  // ROCBTA_CHAIN=classic keeps the step-after-step chain schedule
+  const char* e = getenv("ROCBTA_CHAIN");
+  const bool classic = e && strcmp(e, "classic") == 0;
+  P.fused = (!classic && b % CT == 0) ? 1 : 0;
+  P.trace = nullptr;
   return P;
+}
+
+__global__ void fill_u64_kernel(unsigned long long* p, int64_t count, unsigned long long v);
+
+// Diagnostic: with ROCBTA_CHAIN_TRACE=<file> set, a single-matrix chain
+// launch records its step timeline (chain_trace events) and appends it to
+// <file> as text after the launch (synchronising the stream).
+long long* chain_trace_buffer(size_t count) {
+  static long long* buf = nullptr;
+  static size_t cap = 0;
+  if (count > cap) {
+    if (buf) (void)hipFree(buf);
+    buf = nullptr;
+    cap = 0;
+    if (hipMalloc(&buf, count * sizeof(long long)) != hipSuccess) return nullptr;
+    cap = count;
+  }
+  return buf;
 }
 
 int pobtaf_chain(int64_t n, int64_t b, int64_t a, double* D, double* L, double* R, double* T,
@@ -587,9 +610,32 @@ int pobtaf_chain(int64_t n, int64_t b, int64_t a, double* D, double* L, double* 
   const int64_t nflag = (int64_t)((lay.linv - lay.counter) / sizeof(int));
   fill_i32_kernel<<<(int)std::min<int64_t>(1024, (nflag + 255) / 256), 256, 0, s>>>(P.counter, nflag, 0);
   RB_CHECK(hipGetLastError());
+  const char* trace_path = getenv("ROCBTA_CHAIN_TRACE");
+  const size_t nev = (size_t)CHAIN_TRACE_EV * P.NT;
+  if (trace_path) {
+    P.trace = chain_trace_buffer(nev);
+    if (P.trace) {
+      fill_u64_kernel<<<64, 256, 0, s>>>((unsigned long long*)P.trace, (int64_t)nev, 0ull);
+      RB_CHECK(hipGetLastError());
+    }
+  }
   const int grid = std::max(2, std::min(P.ntasks + 1, 2 * device_cus()));
   pobtaf_chain_kernel<<<grid, CH_THREADS, 0, s>>>(P);
   RB_CHECK(hipGetLastError());
+  if (P.trace) {
+    std::vector<long long> h(nev);
+    RB_CHECK(hipStreamSynchronize(s));
+    RB_CHECK(hipMemcpy(h.data(), P.trace, nev * sizeof(long long), hipMemcpyDeviceToHost));
+    if (FILE* f = fopen(trace_path, "a")) {
+      fprintf(f, "# chain trace n=%lld b=%lld a=%lld NT=%d fused=%d\n", (long long)n, (long long)b,
+              (long long)a, P.NT, P.fused);
+      for (int g = 0; g < P.NT; ++g) {
+        for (int e = 0; e < CHAIN_TRACE_EV; ++e) fprintf(f, "%lld ", h[(size_t)CHAIN_TRACE_EV * g + e]);
+        fprintf(f, "\n");
+      }
+      fclose(f);
+    }
+  }
   return ROCBTA_OK;
 }
 
@@ -653,13 +699,17 @@ constexpr int BS_MAX_G = 1024;  // workgroups of the block sweep (tip partial su
 
 int bs_cw(int64_t r) { return r <= 1 ? 1 : (r <= 4 ? 4 : 8); }
 bool bs_fits(int64_t a, int64_t r) { return a * bs_cw(r) <= BS_TIPMAX; }
+// Wider right-hand sides take the 64-row-tile sweep by default: the block
+// sweep re-reads the factor once per group of 8 columns (measured at n=256,
+// b=1024, a=32: r=8 21 ms, r=64 117 ms against 59 and 58 ms for the sweep).
+constexpr int BS_MAX_R = 8;
 
 int pobtas_algo(int64_t b, int64_t a, int64_t r, int partial) {
   const char* e = getenv("ROCBTA_POBTAS_ALGO");
   if (e && strcmp(e, "loop") == 0) return SOLVE_LOOP;
   if (e && strcmp(e, "sweep") == 0) return partial ? SOLVE_LOOP : SOLVE_SWEEP;
   if (e && strcmp(e, "block") == 0) return bs_fits(a, r) ? SOLVE_BLOCK : (partial ? SOLVE_LOOP : SOLVE_SWEEP);
-  if (b >= BS_MIN_B && bs_fits(a, r)) return SOLVE_BLOCK;
+  if (b >= BS_MIN_B && r <= BS_MAX_R && bs_fits(a, r)) return SOLVE_BLOCK;
   return partial ? SOLVE_LOOP : SOLVE_SWEEP;
 }
 
@@ -791,8 +841,13 @@ BsLayout bs_layout(int64_t n, int64_t b, int64_t a, int64_t r) {
 }
 
 template <int CW>
-int launch_block_sweep(const BlockSweepP& P, hipStream_t s) {
-  pobtas_block_kernel<CW><<<P.G, BS_THREADS, 0, s>>>(P);
+int launch_block_sweep(BlockSweepP P, hipStream_t s) {
+  const int rw = bs_rw(CW, P.trans);
+  P.G = (int)std::min<int64_t>(std::min<int64_t>((P.b + rw - 1) / rw, BS_MAX_G), device_cus());
+  if (P.trans == 0)
+    pobtas_block_kernel<CW, 0><<<P.G, BS_THREADS, 0, s>>>(P);
+  else
+    pobtas_block_kernel<CW, 1><<<P.G, BS_THREADS, 0, s>>>(P);
   RB_CHECK(hipGetLastError());
   return ROCBTA_OK;
 }
@@ -826,7 +881,6 @@ int pobtas_block(int64_t n, int64_t b, int64_t a, const double* D, const double*
   P.a = (int)a;
   P.trans = trans;
   P.partial = partial;
-  P.G = (int)std::min<int64_t>(std::min<int64_t>((b + BS_RW - 1) / BS_RW, BS_MAX_G), device_cus());
   P.sbuf = (double*)(ws + lay.sbuf);
   P.tbuf = (double*)(ws + lay.tbuf);
   P.tpart = (double*)(ws + lay.tpart);

@@ -18,7 +18,7 @@
 // forward, y_{n-1} = t_{n-1} and b_t -= sum_{i<n-1} R_i y_i is left unsolved;
 // backward, x_{n-1} and x_t are the given B rows.
 //
-// Work split: the b rows of a block are cut into row groups of BS_RW rows;
+// Work split: the b rows of a block are cut into row groups of RW rows;
 // workgroup w owns row groups w, w + G, w + 2G, ... of every block (G
 // workgroups, all co-resident).  A row group's phase A needs the whole
 // previous solution block; its forward phase B needs t of the rows up to
@@ -35,8 +35,12 @@
 
 namespace rocbta {
 
-constexpr int BS_RW = 4;  // rows per row group
 constexpr int BS_THREADS = 256;
+// Rows per row group (RW * CW partial sums per thread).  Four rows per
+// group in both directions: measured at n=256, b=1024, r=1, wider groups
+// for the backward sweep's column reads (16 rows, 64 workgroups) took
+// 13.7 ms against 10.1 ms for the backward solve.
+__host__ __device__ constexpr int bs_rw(int cw, int trans) { return 4 + 0 * cw * trans; }
 
 struct BlockSweepP {
   const double* L;     // (n-1, b, b)
@@ -104,14 +108,14 @@ __device__ __forceinline__ void bs_reduce(const double (&acc)[NV], double* red, 
 // Tip entries (m, c) per thread: the block sweep takes a * CW <= BS_TIPMAX.
 constexpr int BS_TIPMAX = 2 * BS_THREADS;
 
-template <int CW>
+template <int CW, int RW>
 struct BsLds {
-  double red[BS_RW * CW * BS_THREADS];
-  double out[BS_THREADS + BS_RW * CW * 4];
+  double red[RW * CW * BS_THREADS];
+  double out[BS_THREADS + RW * CW * 4];
   double xt[BS_TIPMAX];  // x_t (backward) / the tip right-hand side (forward)
 };
 
-// Rows of block i (k0 .. k0 + BS_RW - 1) times the solution block `src`
+// Rows of block i (k0 .. k0 + RW - 1) times the solution block `src`
 // (forward: M[k][j] row access, backward: M[j][k] column access), K range
 // [jlo, jhi), into acc[q][c].  src rows are polled from the buffer: a thread
 // first issues the matrix loads and the polled loads of BS_KU reduction
@@ -119,19 +123,19 @@ struct BsLds {
 // one round trip covers all of them.
 constexpr int BS_KU = 4;
 
-template <int CW, bool TR>
+template <int CW, int RW, bool TR>
 __device__ __forceinline__ void bs_product(const BlockSweepP& P, const double* M, int k0,
                                            const double* src, int jlo, int jhi,
-                                           double (&acc)[BS_RW][CW]) {
+                                           double (&acc)[RW][CW]) {
   const int b = P.b;
   for (int j0 = jlo + (int)threadIdx.x; j0 < jhi; j0 += BS_KU * BS_THREADS) {
-    double m[BS_KU][BS_RW];
+    double m[BS_KU][RW];
     unsigned long long v[BS_KU][CW];
 #pragma unroll
     for (int u = 0; u < BS_KU; ++u) {
       const int j = j0 + u * BS_THREADS;
 #pragma unroll
-      for (int q = 0; q < BS_RW; ++q) {
+      for (int q = 0; q < RW; ++q) {
         const int k = k0 + q;
         m[u][q] = (j < jhi && k < b) ? (TR ? M[(int64_t)j * b + k] : M[(int64_t)k * b + j]) : 0.0;
       }
@@ -161,28 +165,28 @@ __device__ __forceinline__ void bs_product(const BlockSweepP& P, const double* M
 #pragma unroll
     for (int u = 0; u < BS_KU; ++u)
 #pragma unroll
-      for (int q = 0; q < BS_RW; ++q)
+      for (int q = 0; q < RW; ++q)
 #pragma unroll
         for (int c = 0; c < CW; ++c)
           acc[q][c] = fma(m[u][q], __longlong_as_double((long long)v[u][c]), acc[q][c]);
   }
 }
 
-template <int CW>
-__device__ __forceinline__ void bs_zero(double (&acc)[BS_RW][CW]) {
+template <int CW, int RW>
+__device__ __forceinline__ void bs_zero(double (&acc)[RW][CW]) {
 #pragma unroll
-  for (int q = 0; q < BS_RW; ++q)
+  for (int q = 0; q < RW; ++q)
 #pragma unroll
     for (int c = 0; c < CW; ++c) acc[q][c] = 0.0;
 }
 
 // Publish rows k0.. of block i (values in S.out[q * CW + c]) to buf, and
 // optionally write them to B.
-template <int CW>
+template <int CW, int RW>
 __device__ __forceinline__ void bs_emit(const BlockSweepP& P, double* buf, int i, int k0,
                                         const double* vals, bool to_b) {
   const int tid = threadIdx.x;
-  if (tid < BS_RW * CW) {
+  if (tid < RW * CW) {
     const int q = tid / CW, c = tid % CW, k = k0 + q;
     if (k < P.b && c < P.r) {
       const int64_t row = (int64_t)i * P.b + k;
@@ -192,14 +196,15 @@ __device__ __forceinline__ void bs_emit(const BlockSweepP& P, double* buf, int i
   }
 }
 
-template <int CW>
+template <int CW, int TRANS>
 __global__ void __launch_bounds__(BS_THREADS) pobtas_block_kernel(BlockSweepP P) {
-  __shared__ BsLds<CW> S;
+  constexpr int RW = bs_rw(CW, TRANS);
+  __shared__ BsLds<CW, RW> S;
   const int tid = threadIdx.x;
   const int b = P.b, n = P.n, a = P.a;
-  const int ngroups = (b + BS_RW - 1) / BS_RW;
-  double acc[BS_RW][CW];
-  if (P.trans == 0) {
+  const int ngroups = (b + RW - 1) / RW;
+  double acc[RW][CW];
+  if (TRANS == 0) {
     // ------------------------------------------------------------ forward
     // tip partial sums of this workgroup: thread tid owns entries
     // idx = tid + 256 u of the (m, c) pairs, idx < a * CW
@@ -207,42 +212,42 @@ __global__ void __launch_bounds__(BS_THREADS) pobtas_block_kernel(BlockSweepP P)
     const int nsolve = P.partial ? n - 1 : n;
     for (int i = 0; i < n; ++i) {
       for (int g = blockIdx.x; g < ngroups; g += P.G) {
-        const int k0 = g * BS_RW;
+        const int k0 = g * RW;
         // phase A: t = b_i - L_{i,i-1} y_{i-1}
         bs_zero(acc);
         if (i > 0)
-          bs_product<CW, false>(P, P.L + (int64_t)(i - 1) * b * b, k0,
+          bs_product<CW, RW, false>(P, P.L + (int64_t)(i - 1) * b * b, k0,
                                 P.sbuf + (int64_t)(i - 1) * b * CW, 0, b, acc);
-        double flat[BS_RW * CW];
+        double flat[RW * CW];
 #pragma unroll
-        for (int q = 0; q < BS_RW; ++q)
+        for (int q = 0; q < RW; ++q)
 #pragma unroll
           for (int c = 0; c < CW; ++c) flat[q * CW + c] = acc[q][c];
-        bs_reduce<BS_RW * CW>(flat, S.red, S.out);
-        if (tid < BS_RW * CW) {
+        bs_reduce<RW * CW>(flat, S.red, S.out);
+        if (tid < RW * CW) {
           const int q = tid / CW, c = tid % CW, k = k0 + q;
           const double bv = (k < b && c < P.r) ? P.B[((int64_t)i * b + k) * P.ldb + c] : 0.0;
           S.out[tid] = bv - S.out[tid];
         }
         __syncthreads();
         // the last block of a partial solve keeps t as its result
-        bs_emit<CW>(P, P.tbuf, i, k0, S.out, i >= nsolve);
+        bs_emit<CW, RW>(P, P.tbuf, i, k0, S.out, i >= nsolve);
         __syncthreads();
       }
       if (i >= nsolve) continue;
       for (int g = blockIdx.x; g < ngroups; g += P.G) {
-        const int k0 = g * BS_RW;
-        // phase B: y = Linv_i t (t rows 0 .. k0 + BS_RW - 1)
+        const int k0 = g * RW;
+        // phase B: y = Linv_i t (t rows 0 .. k0 + RW - 1)
         bs_zero(acc);
-        bs_product<CW, false>(P, P.Linv + (int64_t)i * b * b, k0, P.tbuf + (int64_t)i * b * CW, 0,
-                              min(b, k0 + BS_RW), acc);
-        double flat[BS_RW * CW];
+        bs_product<CW, RW, false>(P, P.Linv + (int64_t)i * b * b, k0, P.tbuf + (int64_t)i * b * CW, 0,
+                              min(b, k0 + RW), acc);
+        double flat[RW * CW];
 #pragma unroll
-        for (int q = 0; q < BS_RW; ++q)
+        for (int q = 0; q < RW; ++q)
 #pragma unroll
           for (int c = 0; c < CW; ++c) flat[q * CW + c] = acc[q][c];
-        bs_reduce<BS_RW * CW>(flat, S.red, S.out);
-        bs_emit<CW>(P, P.sbuf, i, k0, S.out, true);
+        bs_reduce<RW * CW>(flat, S.red, S.out);
+        bs_emit<CW, RW>(P, P.sbuf, i, k0, S.out, true);
         // tip: b_t -= R_i y_i over this row group
 #pragma unroll
         for (int u = 0; u < BS_TIPMAX / BS_THREADS; ++u) {
@@ -250,7 +255,7 @@ __global__ void __launch_bounds__(BS_THREADS) pobtas_block_kernel(BlockSweepP P)
           if (idx < a * CW) {
             const int m = idx / CW, c = idx % CW;
 #pragma unroll
-            for (int q = 0; q < BS_RW; ++q) {
+            for (int q = 0; q < RW; ++q) {
               const int k = k0 + q;
               if (k < b) tacc[u] = fma(P.R[((int64_t)i * a + m) * b + k], S.out[q * CW + c], tacc[u]);
             }
@@ -322,29 +327,29 @@ __global__ void __launch_bounds__(BS_THREADS) pobtas_block_kernel(BlockSweepP P)
   for (int i = n - 1; i >= 0; --i) {
     const bool skip = P.partial && i == n - 1;  // x_{n-1} is the given B block
     for (int g = blockIdx.x; g < ngroups; g += P.G) {
-      const int k0 = g * BS_RW;
+      const int k0 = g * RW;
       if (skip) {
-        if (tid < BS_RW * CW) {
+        if (tid < RW * CW) {
           const int q = tid / CW, c = tid % CW, k = k0 + q;
           S.out[tid] = (k < b && c < P.r) ? P.B[((int64_t)i * b + k) * P.ldb + c] : 0.0;
         }
         __syncthreads();
-        bs_emit<CW>(P, P.sbuf, i, k0, S.out, false);
+        bs_emit<CW, RW>(P, P.sbuf, i, k0, S.out, false);
         __syncthreads();
         continue;
       }
       // phase A: s = y_i - L_{i+1,i}^T x_{i+1} - R_i^T x_t
       bs_zero(acc);
       if (i + 1 < n)
-        bs_product<CW, true>(P, P.L + (int64_t)i * b * b, k0, P.sbuf + (int64_t)(i + 1) * b * CW,
+        bs_product<CW, RW, true>(P, P.L + (int64_t)i * b * b, k0, P.sbuf + (int64_t)(i + 1) * b * CW,
                              0, b, acc);
-      double flat[BS_RW * CW];
+      double flat[RW * CW];
 #pragma unroll
-      for (int q = 0; q < BS_RW; ++q)
+      for (int q = 0; q < RW; ++q)
 #pragma unroll
         for (int c = 0; c < CW; ++c) flat[q * CW + c] = acc[q][c];
-      bs_reduce<BS_RW * CW>(flat, S.red, S.out);
-      if (tid < BS_RW * CW) {
+      bs_reduce<RW * CW>(flat, S.red, S.out);
+      if (tid < RW * CW) {
         const int q = tid / CW, c = tid % CW, k = k0 + q;
         double v = 0.0;
         if (k < b && c < P.r) {
@@ -354,23 +359,23 @@ __global__ void __launch_bounds__(BS_THREADS) pobtas_block_kernel(BlockSweepP P)
         S.out[tid] = v;
       }
       __syncthreads();
-      bs_emit<CW>(P, P.tbuf, i, k0, S.out, false);
+      bs_emit<CW, RW>(P, P.tbuf, i, k0, S.out, false);
       __syncthreads();
     }
     if (skip) continue;
     for (int g = blockIdx.x; g < ngroups; g += P.G) {
-      const int k0 = g * BS_RW;
+      const int k0 = g * RW;
       // phase B: x = Linv_i^T s (s rows k0 .. b - 1)
       bs_zero(acc);
-      bs_product<CW, true>(P, P.Linv + (int64_t)i * b * b, k0, P.tbuf + (int64_t)i * b * CW, k0, b,
+      bs_product<CW, RW, true>(P, P.Linv + (int64_t)i * b * b, k0, P.tbuf + (int64_t)i * b * CW, k0, b,
                            acc);
-      double flat[BS_RW * CW];
+      double flat[RW * CW];
 #pragma unroll
-      for (int q = 0; q < BS_RW; ++q)
+      for (int q = 0; q < RW; ++q)
 #pragma unroll
         for (int c = 0; c < CW; ++c) flat[q * CW + c] = acc[q][c];
-      bs_reduce<BS_RW * CW>(flat, S.red, S.out);
-      bs_emit<CW>(P, P.sbuf, i, k0, S.out, true);
+      bs_reduce<RW * CW>(flat, S.red, S.out);
+      bs_emit<CW, RW>(P, P.sbuf, i, k0, S.out, true);
       __syncthreads();
     }
   }
