@@ -331,6 +331,7 @@ int rocbta_batched_create(int nmat, const int64_t* n, const int64_t* b, const in
     MatOff& q = mo[m];
     char* ws = base + q.af;
     chains[m] = chain_params(n[m], b[m], a[m], D[m], L[m], R[m], T[m], p->d_info + m, ws, q.lay);
+    chains[m].fused = 0;  // the many-matrix kernel runs the step-after-step chain
     chains[m].tip_inv = a[m] > 0 ? (double*)(base + q.tipinv) : nullptr;
     zero.push_back({(double*)(ws + q.lay.counter),
                     (int64_t)((q.lay.linv - q.lay.counter) / sizeof(double)), 0.0});

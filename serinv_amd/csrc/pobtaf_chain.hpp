@@ -974,16 +974,13 @@ __device__ __forceinline__ void chain_run_fused(const PobtafChain& P, double* Sa
     }
     __syncthreads();
     chain_trace(P, g, 2);
-    // 2. wave 0: block row 3 of Linv (X_33, then X_32, X_31, X_30);
+    // 2. wave 0: X_33 = L(g,g)[48:64, 48:64]^{-1};
     //    wave J+1: column block J of L(g+1,g) = sum over k < 16J + 16 of
     //    PX(:, k) Linv(16J + n, k)
     f64x4 cb[4];
     f32x4 cb32[4];
     if (wave == 0) {
       wave_diag_inv16(S2, X2, 48);
-      __builtin_amdgcn_wave_barrier();
-#pragma unroll
-      for (int J = 2; J >= 0; --J) wave_trtri_offdiag(S2, X2, &X2[48][48], TT_LD, 3, J);
     } else if (!P.f32) {
       if (wave == 1) chain_colblock<0>(Ps, X2, cb);
       else if (wave == 2) chain_colblock<1>(Ps, X2, cb);
@@ -993,20 +990,7 @@ __device__ __forceinline__ void chain_run_fused(const PobtafChain& P, double* Sa
       else if (wave == 2) chain_colblock<1>(Ps, X2, cb32);
       else chain_colblock<2>(Ps, X2, cb32);
     }
-    __syncthreads();  // every read of PX(g)'s column blocks 0-2 is done; Linv(g) is complete
-    if (wave == 1) {
-      // Linv(g) to memory by wave 1 alone (released by it in step 4), two
-      // whole rows per store instruction
-      const int hr = lane >> 5, cc = (lane & 31) * 2;
-#pragma unroll 8
-      for (int k = 0; k < 32; ++k) {
-        const int row = 2 * k + hr;
-        cdbl2 v;
-        v.x = X2[row][cc];
-        v.y = X2[row][cc + 1];
-        *reinterpret_cast<cdbl2*>(li_out + row * CT + cc) = v;
-      }
-    }
+    __syncthreads();  // every read of PX(g)'s column blocks 0-2 is done; X_33 is visible
     if (wave >= 1) {
       const int J = wave - 1;
 #pragma unroll
@@ -1064,8 +1048,23 @@ __device__ __forceinline__ void chain_run_fused(const PobtafChain& P, double* Sa
           Ps[(r0 + acc_row_f32(lane, r)) * TT_LD + 48 + acc_col(lane)] = (double)y[r];
       }
     }
-    __syncthreads();  // L(g+1,g) complete in Ps
+    // the rest of Linv's block row 3: wave J+1 forms X_3J
+    if (wave >= 1) wave_trtri_offdiag(S2, X2, &X2[48][48], TT_LD, 3, wave - 1);
+    __syncthreads();  // L(g+1,g) complete in Ps; Linv(g) complete
     chain_trace(P, g, 4);
+    if (wave == 1) {
+      // Linv(g) to memory by wave 1 alone (released by it in step 4), two
+      // whole rows per store instruction
+      const int hr = lane >> 5, cc = (lane & 31) * 2;
+#pragma unroll 8
+      for (int k = 0; k < 32; ++k) {
+        const int row = 2 * k + hr;
+        cdbl2 v;
+        v.x = X2[row][cc];
+        v.y = X2[row][cc + 1];
+        *reinterpret_cast<cdbl2*>(li_out + row * CT + cc) = v;
+      }
+    }
     lds_tile_store(Ps, TT_LD, xt);
     pending = g;
     // 4. D(g+1,g+1) = PD(g+1) - L(g+1,g) L(g+1,g)^T on the lower quadrants

@@ -121,20 +121,23 @@ def test_batched_more_systems_than_cus(gpu_device):
             assert relerr(g, o) < TOL, f"system {k} X_{nm}: {relerr(g, o):.2e}"
 
 
-def test_batched_mixed(gpu_device):
+def test_batched_mixed(gpu_device, monkeypatch):
     """Mixed FP32/FP64 plan: every system against the FP64 oracle at the bar
     of the mixed form (FP32 products, 1e-5), and its factor equal to the
-    per-matrix mixed factorization (same tile kernels)."""
+    per-matrix mixed factorization with the step-after-step chain schedule
+    (ROCBTA_CHAIN=classic), which runs the same tile kernels as the plan."""
     from serinv_amd.batched import BatchedBTA
     from serinv_amd.logdet import pobta_logdet
     from serinv_amd.mixed import pobtaf_mixed
 
     host, dev = _make(gpu_device, SHAPES, seed0=1500)
     single = []
+    monkeypatch.setenv("ROCBTA_CHAIN", "classic")
     for d in dev:
         f = [t.clone() for t in d[:4]]
         pobtaf_mixed(*f)
         single.append((f, float(pobta_logdet(f[0], f[3]))))
+    monkeypatch.delenv("ROCBTA_CHAIN")
     plan = BatchedBTA([d[:4] for d in dev], rhs=[d[4] for d in dev], selinv=False, graph=True,
                       mixed=True)
     plan.run()
