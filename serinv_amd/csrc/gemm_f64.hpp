@@ -210,17 +210,34 @@ __device__ __forceinline__ void gemm_mfma_stage(const double* XA, const double* 
   const int wave = (threadIdx.x >> 6) & 3;
   const int wm = (wave >> 1) * (TS / 2), wn = (wave & 1) * (TS / 2);
   const int li = lane & 15, lk = lane >> 4;
+  // operand registers double-buffered across k steps: the LDS reads of
+  // step s + 1 are issued before the MFMAs of step s, which hide their
+  // latency
+  constexpr int NS = GTile<TS>::K / 4;
+  double av[2][NB], bv[2][NB];
 #pragma unroll
-  for (int kk = 0; kk < GTile<TS>::K; kk += 4) {
-    double av[NB], bv[NB];
+  for (int i = 0; i < NB; ++i) av[0][i] = slice_at<TS, !TA>(XA, wm + 16 * i + li, lk);
 #pragma unroll
-    for (int i = 0; i < NB; ++i) av[i] = slice_at<TS, !TA>(XA, wm + 16 * i + li, kk + lk);
+  for (int j = 0; j < NB; ++j) bv[0][j] = slice_at<TS, TB>(XB, wn + 16 * j + li, lk);
 #pragma unroll
-    for (int j = 0; j < NB; ++j) bv[j] = slice_at<TS, TB>(XB, wn + 16 * j + li, kk + lk);
+  for (int s = 0; s < NS; ++s) {
+    const int cur = s & 1;
+    if (s + 1 < NS) {
+#pragma unroll
+      for (int i = 0; i < NB; ++i)
+        av[cur ^ 1][i] = slice_at<TS, !TA>(XA, wm + 16 * i + li, 4 * (s + 1) + lk);
+#pragma unroll
+      for (int j = 0; j < NB; ++j)
+        bv[cur ^ 1][j] = slice_at<TS, TB>(XB, wn + 16 * j + li, 4 * (s + 1) + lk);
+    }
+    // keep the reads ahead of the MFMAs (the scheduler would otherwise sink
+    // them to their use and wait on each)
+    __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
     for (int i = 0; i < NB; ++i)
 #pragma unroll
-      for (int j = 0; j < NB; ++j) acc[i][j] = mfma_acc(av[i], bv[j], acc[i][j]);
+      for (int j = 0; j < NB; ++j) acc[i][j] = mfma_acc(av[cur][i], bv[cur][j], acc[i][j]);
+    __builtin_amdgcn_sched_barrier(0);
   }
 }
 

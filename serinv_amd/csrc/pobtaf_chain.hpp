@@ -338,16 +338,27 @@ __device__ __forceinline__ void mfma_abt(const double* XA, const double* XB, ACC
   const int wm = (wave >> 1) * 32, wn = (wave & 1) * 32;
   if (lower_only && wn > wm) return;
   const int li = lane & 15, lk = lane >> 4;
-#pragma unroll 4
+  // operands double-buffered across k steps: the LDS reads of step kk + 4
+  // are issued before the MFMAs of step kk
+  double a[2][2], b[2][2];
+  auto fetch = [&](int buf, int kk) {
+    a[buf][0] = XA[(wm + li) * KP + kk + lk];
+    a[buf][1] = XA[(wm + 16 + li) * KP + kk + lk];
+    b[buf][0] = XB[(wn + li) * KP + kk + lk];
+    b[buf][1] = XB[(wn + 16 + li) * KP + kk + lk];
+  };
+  fetch(0, 0);
+#pragma unroll
   for (int kk = 0; kk < CT; kk += 4) {
-    const double a0 = sgn * XA[(wm + li) * KP + kk + lk];
-    const double a1 = sgn * XA[(wm + 16 + li) * KP + kk + lk];
-    const double b0 = XB[(wn + li) * KP + kk + lk];
-    const double b1 = XB[(wn + 16 + li) * KP + kk + lk];
-    acc[0][0] = mfma_acc(a0, b0, acc[0][0]);
-    acc[0][1] = mfma_acc(a0, b1, acc[0][1]);
-    acc[1][0] = mfma_acc(a1, b0, acc[1][0]);
-    acc[1][1] = mfma_acc(a1, b1, acc[1][1]);
+    const int cur = (kk >> 2) & 1;
+    if (kk + 4 < CT) fetch(cur ^ 1, kk + 4);
+    __builtin_amdgcn_sched_barrier(0);
+    const double a0 = sgn * a[cur][0], a1 = sgn * a[cur][1];
+    acc[0][0] = mfma_acc(a0, b[cur][0], acc[0][0]);
+    acc[0][1] = mfma_acc(a0, b[cur][1], acc[0][1]);
+    acc[1][0] = mfma_acc(a1, b[cur][0], acc[1][0]);
+    acc[1][1] = mfma_acc(a1, b[cur][1], acc[1][1]);
+    __builtin_amdgcn_sched_barrier(0);
   }
 }
 
