@@ -812,7 +812,7 @@ int pobtas_sweep(int64_t n, int64_t b, int64_t a, const double* D, const double*
 }
 
 struct BsLayout {
-  size_t status, linv, w, tinv, wt, sbuf, tbuf, tpart, tipbuf, total;
+  size_t status, linv, lt, w, tinv, wt, sbuf, tbuf, tpart, tipbuf, total;
   int cw;
 };
 
@@ -828,6 +828,7 @@ BsLayout bs_layout(int64_t n, int64_t b, int64_t a, int64_t r) {
   const int64_t ntb = (b + TT - 1) / TT, nta = (a + TT - 1) / TT;
   l.status = take(sizeof(int));
   l.linv = take((size_t)n * b * b * sizeof(double));
+  l.lt = take((size_t)std::max<int64_t>(n - 1, 1) * b * b * sizeof(double));  // backward: L^T
   // trtri scratch: 64-wide levels need ntb * 64^2 per block, halving b^2 / 4
   l.w = take((size_t)n * std::max<int64_t>(ntb * TT * TT, ntb * TT * ntb * TT / 4) * sizeof(double));
   l.tinv = take((size_t)a * a * sizeof(double));
@@ -870,8 +871,25 @@ int pobtas_block(int64_t n, int64_t b, int64_t a, const double* D, const double*
     rc = block_trtri(T, a, 0, Tinv, a, a * a, (int)a, 1, (double*)(ws + lay.wt), s);
     if (rc) return rc;
   }
+  if (trans) {
+    // the backward sweep reads L_{i+1,i}^T and Linv_i^T by rows
+    const int T = (int)((b + 63) / 64);
+    for (int64_t z0 = 0; z0 < n; z0 += 65535) {
+      const int nz = (int)std::min<int64_t>(65535, n - z0);
+      transpose_blocks_inplace_kernel<<<dim3(T * (T + 1) / 2, nz), 256, 0, s>>>(
+          Linv + z0 * b * b, (int)b, b * b);
+      RB_CHECK(hipGetLastError());
+    }
+    for (int64_t z0 = 0; z0 + 1 < n; z0 += 65535) {
+      const int nz = (int)std::min<int64_t>(65535, n - 1 - z0);
+      transpose_blocks_kernel<<<dim3(T * T, nz), 256, 0, s>>>(L + z0 * b * b,
+                                                             (double*)(ws + lay.lt) + z0 * b * b,
+                                                             (int)b, b * b, T);
+      RB_CHECK(hipGetLastError());
+    }
+  }
   BlockSweepP P{};
-  P.L = L;
+  P.L = trans ? (const double*)(ws + lay.lt) : L;
   P.R = R;
   P.Linv = Linv;
   P.Tinv = Tinv;

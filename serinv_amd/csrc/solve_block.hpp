@@ -18,6 +18,11 @@
 // forward, y_{n-1} = t_{n-1} and b_t -= sum_{i<n-1} R_i y_i is left unsolved;
 // backward, x_{n-1} and x_t are the given B rows.
 //
+// The backward sweep reads the transposes of L_{i+1,i} and Linv_i, written
+// beforehand (transpose_blocks_kernel), so that both directions read whole
+// matrix rows: with the factor's own layout every output row of the
+// backward products is a matrix column, read in 8-byte pieces.
+//
 // Work split: the b rows of a block are cut into row groups of RW rows;
 // workgroup w owns row groups w, w + G, w + 2G, ... of every block (G
 // workgroups, all co-resident).  A row group's phase A needs the whole
@@ -36,16 +41,66 @@
 namespace rocbta {
 
 constexpr int BS_THREADS = 256;
-// Rows per row group (RW * CW partial sums per thread).  Four rows per
-// group in both directions: measured at n=256, b=1024, r=1, wider groups
-// for the backward sweep's column reads (16 rows, 64 workgroups) took
-// 13.7 ms against 10.1 ms for the backward solve.
+// Rows per row group (RW * CW partial sums per thread): four in both
+// directions.
 __host__ __device__ constexpr int bs_rw(int cw, int trans) { return 4 + 0 * cw * trans; }
 
+// Transposes of nb square b x b blocks, 64 x 64 tiles through LDS (pitch 65,
+// so that the column reads do not share banks).  Out of place: block z of
+// src (stride sblk) -> block z of dst; workgroup (tile (I, J), z).
+__global__ void __launch_bounds__(256) transpose_blocks_kernel(const double* src, double* dst,
+                                                               int b, int64_t sblk, int T) {
+  __shared__ double t[64][65];
+  const int I = blockIdx.x % T, J = blockIdx.x / T;
+  const int64_t off = (int64_t)blockIdx.y * sblk;
+  const int c = threadIdx.x & 63, r0 = threadIdx.x >> 6;
+#pragma unroll
+  for (int q = 0; q < 16; ++q) {
+    const int r = r0 + 4 * q, row = 64 * I + r, col = 64 * J + c;
+    t[r][c] = (row < b && col < b) ? src[off + (int64_t)row * b + col] : 0.0;
+  }
+  __syncthreads();
+#pragma unroll
+  for (int q = 0; q < 16; ++q) {
+    const int r = r0 + 4 * q, row = 64 * J + r, col = 64 * I + c;
+    if (row < b && col < b) dst[off + (int64_t)row * b + col] = t[c][r];
+  }
+}
+
+// In place: workgroup (pair p, z) swaps the transposes of tiles (I, J) and
+// (J, I), I <= J, p = J (J + 1) / 2 + I; a diagonal tile is transposed
+// within itself.
+__global__ void __launch_bounds__(256) transpose_blocks_inplace_kernel(double* X, int b,
+                                                                       int64_t sblk) {
+  __shared__ double t1[64][65], t2[64][65];
+  int J = (int)((__builtin_sqrtf(8.0f * (float)blockIdx.x + 1.0f) - 1.0f) * 0.5f);
+  while ((J + 1) * (J + 2) / 2 <= (int)blockIdx.x) ++J;
+  while (J * (J + 1) / 2 > (int)blockIdx.x) --J;
+  const int I = (int)blockIdx.x - J * (J + 1) / 2;
+  double* x = X + (int64_t)blockIdx.y * sblk;
+  const int c = threadIdx.x & 63, r0 = threadIdx.x >> 6;
+#pragma unroll
+  for (int q = 0; q < 16; ++q) {
+    const int r = r0 + 4 * q;
+    const int ra = 64 * I + r, ca = 64 * J + c, rb = 64 * J + r, cb = 64 * I + c;
+    t1[r][c] = (ra < b && ca < b) ? x[(int64_t)ra * b + ca] : 0.0;
+    t2[r][c] = (rb < b && cb < b) ? x[(int64_t)rb * b + cb] : 0.0;
+  }
+  __syncthreads();
+#pragma unroll
+  for (int q = 0; q < 16; ++q) {
+    const int r = r0 + 4 * q;
+    const int ra = 64 * I + r, ca = 64 * J + c, rb = 64 * J + r, cb = 64 * I + c;
+    if (rb < b && cb < b) x[(int64_t)rb * b + cb] = t1[c][r];   // tile (J, I) = tile(I, J)^T
+    if (I != J && ra < b && ca < b) x[(int64_t)ra * b + ca] = t2[c][r];
+  }
+}
+
 struct BlockSweepP {
-  const double* L;     // (n-1, b, b)
+  const double* L;     // (n-1, b, b); backward: the transposes L_{i+1,i}^T
   const double* R;     // (n, a, b)
-  const double* Linv;  // (n, b, b): inverses of the diagonal blocks (dense, zero above)
+  const double* Linv;  // (n, b, b): inverses of the diagonal blocks (dense, zero above);
+                       // backward: their transposes (zero below)
   const double* Tinv;  // (a, a): inverse of the tip factor (zero above)
   double* B;           // right-hand sides, columns [0, r), row stride ldb
   int64_t ldb;
@@ -113,6 +168,7 @@ struct BsLds {
   double red[RW * CW * BS_THREADS];
   double out[BS_THREADS + RW * CW * 4];
   double xt[BS_TIPMAX];  // x_t (backward) / the tip right-hand side (forward)
+  double rv[BS_TIPMAX * RW];  // backward: R_i[m][k0 + q] at m * RW + q (a <= BS_TIPMAX)
 };
 
 // Rows of block i (k0 .. k0 + RW - 1) times the solution block `src`
@@ -338,11 +394,18 @@ __global__ void __launch_bounds__(BS_THREADS) pobtas_block_kernel(BlockSweepP P)
         __syncthreads();
         continue;
       }
-      // phase A: s = y_i - L_{i+1,i}^T x_{i+1} - R_i^T x_t
+      // phase A: s = y_i - L_{i+1,i}^T x_{i+1} - R_i^T x_t (P.L holds L^T).
+      // The R_i entries of this row group are fetched by all threads at
+      // once into LDS (ordered by the barriers of bs_reduce), not one after
+      // the other inside the m loop below.
+      for (int idx = tid; idx < a * RW; idx += BS_THREADS) {
+        const int m = idx / RW, q = idx % RW, k = k0 + q;
+        S.rv[idx] = (k < b) ? P.R[((int64_t)i * a + m) * b + k] : 0.0;
+      }
       bs_zero(acc);
       if (i + 1 < n)
-        bs_product<CW, RW, true>(P, P.L + (int64_t)i * b * b, k0, P.sbuf + (int64_t)(i + 1) * b * CW,
-                             0, b, acc);
+        bs_product<CW, RW, false>(P, P.L + (int64_t)i * b * b, k0, P.sbuf + (int64_t)(i + 1) * b * CW,
+                              0, b, acc);
       double flat[RW * CW];
 #pragma unroll
       for (int q = 0; q < RW; ++q)
@@ -354,7 +417,7 @@ __global__ void __launch_bounds__(BS_THREADS) pobtas_block_kernel(BlockSweepP P)
         double v = 0.0;
         if (k < b && c < P.r) {
           v = P.B[((int64_t)i * b + k) * P.ldb + c] - S.out[tid];
-          for (int m = 0; m < a; ++m) v = fma(-P.R[((int64_t)i * a + m) * b + k], S.xt[m * CW + c], v);
+          for (int m = 0; m < a; ++m) v = fma(-S.rv[m * RW + q], S.xt[m * CW + c], v);
         }
         S.out[tid] = v;
       }
@@ -367,8 +430,8 @@ __global__ void __launch_bounds__(BS_THREADS) pobtas_block_kernel(BlockSweepP P)
       const int k0 = g * RW;
       // phase B: x = Linv_i^T s (s rows k0 .. b - 1)
       bs_zero(acc);
-      bs_product<CW, RW, true>(P, P.Linv + (int64_t)i * b * b, k0, P.tbuf + (int64_t)i * b * CW, k0, b,
-                           acc);
+      bs_product<CW, RW, false>(P, P.Linv + (int64_t)i * b * b, k0, P.tbuf + (int64_t)i * b * CW, k0,
+                            b, acc);
       double flat[RW * CW];
 #pragma unroll
       for (int q = 0; q < RW; ++q)
