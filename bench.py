@@ -178,7 +178,7 @@ def bench_batched(dev, nmat=64, r=4, reps=3, tune=True, mixed=False):
     import torch
 
     import serinv_amd.algs as algs
-    from serinv_amd.autotune import tune_batched
+    from serinv_amd.autotune import apply_tuning, tune_batched
     from serinv_amd.batched import BatchedBTA
     from serinv_amd.utils.matrices import bta_flops, bta_solve_flops, bta_spd
 
@@ -224,10 +224,12 @@ def bench_batched(dev, nmat=64, r=4, reps=3, tune=True, mixed=False):
     if tune:
         tuned = tune_batched([w[:4] for w in work], rhs=[w[4] for w in work], selinv=True,
                              use_cache=False, mixed=mixed)
-    plan = BatchedBTA([w[:4] for w in work], rhs=[w[4] for w in work], selinv=True, graph=True,
-                      mixed=mixed)
     if tuned:
-        plan.set_grid(tuned["factor_grid"], tuned["solve_grid"])
+        plan = apply_tuning([w[:4] for w in work], tuned, rhs=[w[4] for w in work], selinv=True,
+                            graph=True, mixed=mixed)
+    else:
+        plan = BatchedBTA([w[:4] for w in work], rhs=[w[4] for w in work], selinv=True, graph=True,
+                          mixed=mixed)
     restore()
     plan.run()  # capture + first replay
     torch.cuda.synchronize(dev)

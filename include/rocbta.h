@@ -163,6 +163,16 @@ int rocbta_dquad(int64_t rows, int64_t r, const double* y, const double* x, int6
  * FP64 tile potrf / trtri, tip and log-determinant); the solves stay FP64
  * with that factor. */
 #define ROCBTA_BATCHED_MIXED 8
+/* Launch shapes (no counterpart in the reference; set by the autotuner):
+ * ROCBTA_BATCHED_RC1 / _RC4 / _RC8 force the solve sweeps' column-chunk
+ * width (right-hand-side columns per chain workgroup; default: 1 for r = 1,
+ * 4 for r <= 4, else 8), ROCBTA_BATCHED_GEMM512 runs the selected-inversion
+ * GEMM phases on 512-thread workgroups whose two halves split each 64x64
+ * tile's K range (default: 256 threads, two workgroups per CU). */
+#define ROCBTA_BATCHED_RC1 16
+#define ROCBTA_BATCHED_RC4 32
+#define ROCBTA_BATCHED_RC8 64
+#define ROCBTA_BATCHED_GEMM512 128
 typedef void* rocbta_batched_plan;
 
 /* Build a plan (allocates its own device workspace; not stream-ordered).

@@ -17,6 +17,8 @@ ROCBTA_ERR_HIP = -2
 ROCBTA_ERR_WORKSPACE = -3
 OP_POBTAF, OP_POBTASI, OP_POBTAS, OP_RESIDUAL, OP_LOGDET = 0, 1, 2, 3, 4
 BATCHED_FACTOR, BATCHED_SOLVE, BATCHED_SELINV, BATCHED_MIXED = 1, 2, 4, 8
+BATCHED_RC = {1: 16, 4: 32, 8: 64}  # forced sweep column-chunk width
+BATCHED_GEMM512 = 128
 
 _ERRORS = {
     ROCBTA_ERR_ARG: "invalid argument",

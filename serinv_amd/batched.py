@@ -15,6 +15,12 @@ sequence is captured into a hipGraph on the first run and replayed after.
 With mixed=True the factorization and the selected inversion run in the
 mixed FP32/FP64 form of serinv_amd.mixed (FP32 tile products; FP64 tile
 potrf/trtri, tip, log-determinant and solves).
+
+Launch shapes (picked per batch class by serinv_amd.autotune): sweep_width
+forces the solve sweeps' column-chunk width (1, 4 or 8 right-hand-side
+columns per chain workgroup; None: 1 for one column, 4 up to four, else 8),
+gemm_threads (256 or 512) the workgroup size of the selected-inversion GEMM
+phases (512: two halves split each 64 x 64 tile's K range).
 """
 
 import ctypes
@@ -27,9 +33,14 @@ from .algs._common import _tensor, bta_shapes
 
 
 class BatchedBTA:
-    def __init__(self, systems, rhs=None, selinv=True, graph=True, mixed=False):
+    def __init__(self, systems, rhs=None, selinv=True, graph=True, mixed=False,
+                 sweep_width=None, gemm_threads=None):
         if not systems:
             raise ValueError("need at least one system")
+        if sweep_width not in (None, 1, 4, 8):
+            raise ValueError(f"sweep_width must be 1, 4 or 8 (got {sweep_width})")
+        if gemm_threads not in (None, 256, 512):
+            raise ValueError(f"gemm_threads must be 256 or 512 (got {gemm_threads})")
         self._keep = []
         ns, bs, as_, rs = [], [], [], []
         ptrs = {k: [] for k in "DLRTB"}
@@ -77,6 +88,10 @@ class BatchedBTA:
             ops |= _lib.BATCHED_SELINV
         if mixed:
             ops |= _lib.BATCHED_MIXED
+        if sweep_width is not None:
+            ops |= _lib.BATCHED_RC[sweep_width]
+        if gemm_threads == 512:
+            ops |= _lib.BATCHED_GEMM512
         I64 = ctypes.c_int64 * nmat
         P = ctypes.c_void_p * nmat
         self._arrays = [I64(*ns), I64(*bs), I64(*as_), I64(*(rs or [0] * nmat))]

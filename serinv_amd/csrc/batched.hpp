@@ -65,6 +65,7 @@ struct BatchedPlan {
   int64_t nempty = 0;
   int sweep_rc = 1;
   int sweep_grid = 0;
+  int gemm512 = 0;  // selected-inversion GEMM phases on 512-thread workgroups
   // selected inversion
   std::vector<PhaseLaunch> phases;
   // graph
@@ -189,8 +190,12 @@ int batched_enqueue(const BatchedPlan& p, hipStream_t s) {
         RB_CHECK(hipGetLastError());
       }
       if (ph.ngemm && ph.gemm_tiles > 0) {
-        gemm_list_kernel<<<ph.gemm_tiles, GT_THREADS, 0, s>>>(
-            (const GemmProblem*)(p.desc + ph.gemm_off), ph.ngemm);
+        if (p.gemm512)
+          gemm_list_kernel_h2<<<ph.gemm_tiles, 2 * GT_THREADS, 0, s>>>(
+              (const GemmProblem*)(p.desc + ph.gemm_off), ph.ngemm);
+        else
+          gemm_list_kernel<<<ph.gemm_tiles, GT_THREADS, 0, s>>>(
+              (const GemmProblem*)(p.desc + ph.gemm_off), ph.ngemm);
         RB_CHECK(hipGetLastError());
       }
     }
@@ -227,8 +232,15 @@ int rocbta_batched_create(int nmat, const int64_t* n, const int64_t* b, const in
     if (solve && (r[m] < 1 || !B[m])) return ROCBTA_ERR_ARG;
     if (solve) rmax = std::max(rmax, r[m]);
   }
+  // sweep chunk width: forced by the launch-shape bits, else by rmax
+  const int rc_bits = ops & (ROCBTA_BATCHED_RC1 | ROCBTA_BATCHED_RC4 | ROCBTA_BATCHED_RC8);
+  if (rc_bits & (rc_bits - 1)) return ROCBTA_ERR_ARG;  // at most one width
+  const int rc_plan = rc_bits == ROCBTA_BATCHED_RC1   ? 1
+                      : rc_bits == ROCBTA_BATCHED_RC4 ? 4
+                      : rc_bits == ROCBTA_BATCHED_RC8 ? 8
+                                                      : sweep_rc(rmax);
   // one matrix's column chunks must fit one sweep wave
-  if (solve && cdiv(rmax, sweep_rc(rmax)) > SW_MAX_CHAINS) return ROCBTA_ERR_ARG;
+  if (solve && cdiv(rmax, rc_plan) > SW_MAX_CHAINS) return ROCBTA_ERR_ARG;
   // chain_params and the recorded GEMM problems read the mixed flag
   MixedScope mixed((ops & ROCBTA_BATCHED_MIXED) != 0);
   BatchedPlan* p = new BatchedPlan();
@@ -238,7 +250,8 @@ int rocbta_batched_create(int nmat, const int64_t* n, const int64_t* b, const in
     delete p;
     return ROCBTA_ERR_HIP;
   }
-  p->sweep_rc = sweep_rc(rmax);
+  p->sweep_rc = rc_plan;
+  p->gemm512 = (ops & ROCBTA_BATCHED_GEMM512) ? 1 : 0;
   const int RC = p->sweep_rc;
   // ---- layout of the workspace arena (offsets first, one allocation)
   size_t off = 0;

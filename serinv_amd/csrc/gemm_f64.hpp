@@ -530,4 +530,18 @@ __global__ void __launch_bounds__(GT_THREADS, 2) gemm_list_kernel(const GemmProb
   gemm_tile<64>(probs[lo], tile, lds);
 }
 
+// The list form with 512-thread workgroups (the halves split each tile's K
+// range, as in gemm_group_kernel_h2).
+__global__ void __launch_bounds__(2 * GT_THREADS, 1) gemm_list_kernel_h2(const GemmProblem* probs,
+                                                                         int nprob) {
+  __shared__ __attribute__((aligned(16))) double lds[8 * GTile<64>::SLICE];
+  const int tile = blockIdx.x;
+  int lo = 0, hi = nprob - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (probs[mid].tile_begin <= tile) lo = mid; else hi = mid - 1;
+  }
+  gemm_tile<64, 2>(probs[lo], tile, lds);
+}
+
 }  // namespace rocbta

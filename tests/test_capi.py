@@ -69,3 +69,16 @@ def test_pobtaf_dag_schedule_selfcheck(lib, a):
             assert lib.rocbta_dag_selfcheck(n, b, a) == 0, (n, b, a)
     assert lib.rocbta_dag_selfcheck(256, 1024, 32) == 0
     assert lib.rocbta_dag_selfcheck(4, 8, 65) == -1  # arrow wider than one tile: not this kernel
+
+
+def test_batched_launch_shape_bits_checked(lib):
+    """Two forced sweep widths at once are rejected before any device work."""
+    n = (ctypes.c_int64 * 1)(3)
+    b = (ctypes.c_int64 * 1)(64)
+    a = (ctypes.c_int64 * 1)(4)
+    r = (ctypes.c_int64 * 1)(2)
+    fake = (ctypes.c_void_p * 1)(0x1000)  # never dereferenced: the call returns first
+    plan = ctypes.c_void_p()
+    ops = _lib.BATCHED_FACTOR | _lib.BATCHED_SOLVE | _lib.BATCHED_RC[1] | _lib.BATCHED_RC[4]
+    rc = lib.rocbta_batched_create(1, n, b, a, fake, fake, fake, fake, fake, r, ops, ctypes.byref(plan))
+    assert rc == _lib.ROCBTA_ERR_ARG and not plan.value

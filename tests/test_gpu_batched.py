@@ -167,3 +167,51 @@ def test_batched_mixed(gpu_device, monkeypatch):
         _, _, X = _oracle(*h)
         for nm, g, o in zip("DLRT", to_host(*d[:4]), X):
             assert relerr(g, o) < 1e-5, f"system {k} X_{nm}: {relerr(g, o):.2e}"
+
+
+@pytest.mark.parametrize("sweep_width,gemm_threads", [(1, 512), (8, 256), (4, 512)])
+def test_batched_launch_shapes(gpu_device, sweep_width, gemm_threads):
+    """Forced tile shapes (sweep column-chunk width, GEMM workgroup size)
+    against the oracle at the FP64 bar."""
+    from serinv_amd.batched import BatchedBTA
+
+    host, dev = _make(gpu_device, SHAPES, seed0=1700)
+    plan = BatchedBTA([d[:4] for d in dev], rhs=[d[4] for d in dev], selinv=True, graph=True,
+                      sweep_width=sweep_width, gemm_threads=gemm_threads)
+    plan.run()
+    info, logdet, status = plan.results()
+    plan.close()
+    assert status == 0 and np.all(info == 0), (info, status)
+    for k, (h, d) in enumerate(zip(host, dev)):
+        ld, x, X = _oracle(*h)
+        assert abs(logdet[k] - ld) <= TOL * abs(ld), f"system {k} logdet"
+        assert relerr(to_host(d[4])[0], x) < TOL, f"system {k} solve"
+        for nm, g, o in zip("DLRT", to_host(*d[:4]), X):
+            assert relerr(g, o) < TOL, f"system {k} X_{nm}: {relerr(g, o):.2e}"
+
+
+def test_autotuner_round_trip(gpu_device):
+    """tune_batched times grids then tile shapes; the tuned plan matches the
+    oracle."""
+    import torch
+
+    from serinv_amd.autotune import apply_tuning, tune_batched
+
+    host, dev = _make(gpu_device, SHAPES[:4], seed0=1800)
+    cus = torch.cuda.get_device_properties(gpu_device).multi_processor_count
+    tuned = tune_batched([d[:4] for d in dev], rhs=[d[4] for d in dev], selinv=True, reps=1,
+                         use_cache=False, candidates=[(cus, cus), (2 * cus, cus)])
+    assert tuned["sweep_width"] in (None, 1, 4, 8) and tuned["gemm_threads"] in (None, 256, 512)
+    shapes = {(t["sweep_width"], t["gemm_threads"]) for t in tuned["trials"]}
+    assert (4, 512) in shapes and (1, 256) in shapes  # r = 9 max: widths 1, 4, 8
+    plan = apply_tuning([d[:4] for d in dev], tuned, rhs=[d[4] for d in dev], selinv=True)
+    plan.run()
+    info, logdet, status = plan.results()
+    plan.close()
+    assert status == 0 and np.all(info == 0)
+    for k, (h, d) in enumerate(zip(host, dev)):
+        ld, x, X = _oracle(*h)
+        assert abs(logdet[k] - ld) <= TOL * abs(ld)
+        assert relerr(to_host(d[4])[0], x) < TOL
+        for g, o in zip(to_host(*d[:4]), X):
+            assert relerr(g, o) < TOL
