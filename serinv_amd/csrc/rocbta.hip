@@ -701,7 +701,8 @@ int bs_cw(int64_t r) { return r <= 1 ? 1 : (r <= 4 ? 4 : 8); }
 bool bs_fits(int64_t a, int64_t r) { return a * bs_cw(r) <= BS_TIPMAX; }
 // Wider right-hand sides take the 64-row-tile sweep by default: the block
 // sweep re-reads the factor once per group of 8 columns (measured at n=256,
-// b=1024, a=32: r=8 21 ms, r=64 117 ms against 59 and 58 ms for the sweep).
+// b=1024, a=32, N+C: r=8 19.8 ms, r=64 89 ms against 59 and 58 ms for the
+// sweep).
 constexpr int BS_MAX_R = 8;
 
 int pobtas_algo(int64_t b, int64_t a, int64_t r, int partial) {
