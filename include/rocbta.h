@@ -40,6 +40,9 @@ extern "C" {
 #define ROCBTA_OP_POBTAS 2
 #define ROCBTA_OP_RESIDUAL 3
 #define ROCBTA_OP_LOGDET 4
+#define ROCBTA_OP_POBTAF_PERMUTED 5
+#define ROCBTA_OP_POBTASI_PERMUTED 6
+#define ROCBTA_OP_POBTAS_PERMUTED 7
 
 /* Library version string. */
 const char* rocbta_version(void);
@@ -197,6 +200,54 @@ int rocbta_batched_results(rocbta_batched_plan plan, int* info, double* logdet, 
 /* Number of kernel launches in one run. */
 int rocbta_batched_launch_count(rocbta_batched_plan plan);
 int rocbta_batched_destroy(rocbta_batched_plan plan);
+
+/* --------------------------------------------------------------- permuted
+ * The "middle partition" forms used by serinv's distributed solvers: the
+ * first and last diagonal blocks and the tip are left to a reduced system,
+ * and the top buffer (n, b, b), buffer[i] = the (0, i) block, carries the
+ * fill-in between block 0 and the middle blocks.  buffer[0] is not used.
+ * a == 0 gives the BT forms (R and T may then be NULL).
+ *
+ * Factorization of the middle blocks 1 .. n-2, in place: D[i], L[i], R[i]
+ * and buffer[i] become the factor blocks L_ii, L_{i+1,i}, L_{t,i}, L_{0,i};
+ * D[0], D[n-1], R[0], R[n-1], T and buffer[n-1] receive the Schur
+ * complements of the reduced system (full blocks, both triangles).  The input
+ * buffer is overwritten (buffer[1] = L[0]^T is set here).
+ * Replaces serinv.algs.pobtaf(..., buffer=) / _pobtaf_permuted
+ * (src/serinv/algs/pobtaf.py:186-281) and, with a == 0, pobtf(..., buffer=)
+ * / _pobtf_permuted (pobtf.py:125-185).
+ * Workspace: rocbta_workspace_bytes(ROCBTA_OP_POBTAF_PERMUTED, n, b, a, 0). */
+int rocbta_dpobtaf_permuted(int64_t n, int64_t b, int64_t a, double* D, double* L, double* R,
+                            double* T, double* buffer, int* info, void* workspace,
+                            size_t workspace_bytes, void* stream);
+/* Selected inversion of the middle blocks, in place, given the factor from
+ * rocbta_dpobtaf_permuted with D[0], D[n-1], R[0], R[n-1], T and buffer[n-1]
+ * already replaced by the inverse's blocks from the reduced system
+ * (buffer[n-1] = X_{0,n-1}).  On exit D, L, R, T hold the selected inverse's
+ * blocks as rocbta_dpobtasi writes them (L[0] = buffer[1]^T = X_{1,0}) and
+ * buffer[i] = X_{0,i}.
+ * Replaces serinv.algs.pobtasi(..., buffer=) / _pobtasi_permuted
+ * (src/serinv/algs/pobtasi.py:175-244) and, with a == 0, pobtsi(...,
+ * buffer=) / _pobtsi_permuted (pobtsi.py:128-179).
+ * Workspace: rocbta_workspace_bytes(ROCBTA_OP_POBTASI_PERMUTED, n, b, a, 0). */
+int rocbta_dpobtasi_permuted(int64_t n, int64_t b, int64_t a, double* D, double* L, double* R,
+                             double* T, double* buffer, void* workspace, size_t workspace_bytes,
+                             void* stream);
+/* Triangular solves over the middle blocks with that factor, in place on B:
+ * trans 'N' runs the forward elimination of blocks 1 .. n-2 (updating B's
+ * first block, block n-1 and tip rows), 'T' / 'C' the backward solve of
+ * blocks n-2 .. 1 (reading the reduced system's solution in B's first block,
+ * block n-1 and tip rows).
+ * Replaces serinv.algs.pobtas(..., buffer=) / _pobtas_permuted
+ * (src/serinv/algs/pobtas.py:164-218) and, with a == 0, pobts(..., buffer=)
+ * / _pobts_permuted (pobts.py:121-165).
+ * Workspace: rocbta_workspace_bytes(ROCBTA_OP_POBTAS_PERMUTED, n, b, a, r)
+ * (its first int32 is set to 0, the status word of rocbta_dpobtas; T is not
+ * read and may be NULL). */
+int rocbta_dpobtas_permuted(int64_t n, int64_t b, int64_t a, const double* D, const double* L,
+                            const double* R, const double* T, const double* buffer, double* B,
+                            int64_t r, int64_t ldb, char trans, void* workspace,
+                            size_t workspace_bytes, void* stream);
 
 /* Diagnostic (no counterpart in the reference): runs the task decoding and
  * dependency enumeration of the persistent pobtaf kernel on the HOST for the

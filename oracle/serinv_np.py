@@ -9,8 +9,9 @@ reference layout:
     lower arrow blocks       (n, a, b)
     arrow tip block          (a, a)
     right-hand side B        (n*b + a, r)   (n*b, r) for the BT solve
-and follows the natural-ordering, non-streaming reference routine step for
-step.  The Cholesky is scipy.linalg.cholesky(lower=True, overwrite_a=False,
+and follows the non-streaming reference routine step for step: the natural
+ordering, or with buffer= the permuted ("middle partition") one, whose top
+buffer (n, b, b) holds buffer[i] = the (0, i) block.  The Cholesky is scipy.linalg.cholesky(lower=True, overwrite_a=False,
 check_finite=False), exactly the function the reference selects for NumPy
 arrays (src/serinv/__init__.py:137-155, line 151); it zero-fills the upper
 triangle.  Triangular solves are scipy.linalg.solve_triangular with the same
@@ -38,10 +39,8 @@ def _check_kwargs(kwargs, allowed):
 
 
 def _reject_unsupported(kwargs):
-    # The oracle restates only the natural, in-memory branch
-    # (src/serinv/algs/pobtaf.py:87-94 and the matching branches elsewhere).
-    if kwargs.get("buffer", None) is not None:
-        raise NotImplementedError("permuted (buffer=...) variants are not restated")
+    # The oracle restates the in-memory branches (src/serinv/algs/pobtaf.py:
+    # 58-94 and the matching branches elsewhere), not the streaming ones.
     if kwargs.get("device_streaming", False):
         raise NotImplementedError("device streaming is not restated")
 
@@ -56,6 +55,8 @@ def pobtaf(D, L, R, T, **kwargs):
     """
     _check_kwargs(kwargs, {"device_streaming", "buffer", "factorize_last_block"})
     _reject_unsupported(kwargs)
+    if kwargs.get("buffer", None) is not None:  # pobtaf.py:59-76
+        return pobtaf_permuted(D, L, R, T, kwargs["buffer"])
     factorize_last_block = kwargs.get("factorize_last_block", True)
     n = D.shape[0]
     for i in range(n - 1):  # pobtaf.py:115
@@ -84,6 +85,8 @@ def pobtasi(D, L, R, T, **kwargs):
     """
     _check_kwargs(kwargs, {"device_streaming", "buffer", "invert_last_block"})
     _reject_unsupported(kwargs)
+    if kwargs.get("buffer", None) is not None:  # pobtasi.py:54-71
+        return pobtasi_permuted(D, L, R, T, kwargs["buffer"])
     invert_last_block = kwargs.get("invert_last_block", True)
     n, b = D.shape[0], D.shape[1]
     eye_b = np.eye(b, dtype=D.dtype)  # pobtasi.py:112
@@ -110,8 +113,10 @@ def pobtas(D, L, R, T, B, trans="N", **kwargs):
     The reference does not validate kwargs for pobtas.
     """
     partial_ = kwargs.get("partial", False)
-    if kwargs.get("buffer", None) is not None:
-        raise NotImplementedError("permuted (buffer=...) variant is not restated")
+    if kwargs.get("buffer", None) is not None:  # pobtas.py:31-47
+        if kwargs.get("device_streaming", False):
+            raise NotImplementedError("Streaming is not implemented for the permuted arrowhead.")
+        return pobtas_permuted(D, L, R, T, B, kwargs["buffer"], trans)
     if kwargs.get("device_streaming", False):
         # pobtas.py:49-52
         raise NotImplementedError("Streaming is not implemented for the natural arrowhead.")
@@ -145,12 +150,85 @@ def pobtas(D, L, R, T, B, trans="N", **kwargs):
 
 
 # --------------------------------------------------------------------------
+# Permuted ("middle partition") forms: blocks 1 .. n-2 are eliminated; block
+# 0, block n-1 and the tip form the reduced system.  a == 0 (R of shape
+# (n, 0, b), T of shape (0, 0)) gives the BT forms pobtf / pobtsi / pobts
+# with buffer=, whose steps are the same without the arrow terms.
+# --------------------------------------------------------------------------
+def pobtaf_permuted(D, L, R, T, buffer):
+    """Follows src/serinv/algs/pobtaf.py:186-281 (_pobtaf_permuted) and, for
+    a == 0, pobtf.py:125-185 (_pobtf_permuted)."""
+    n = D.shape[0]
+    buffer[1] = L[0].conj().T  # pobtaf.py:204
+    for i in range(1, n - 1):  # :207
+        D[i] = _cholesky(D[i])  # :209
+        L[i] = sla.solve_triangular(D[i], L[i].conj().T, lower=True).conj().T  # :213-221
+        buffer[i] = sla.solve_triangular(D[i], buffer[i].conj().T, lower=True).conj().T  # :224-232
+        R[i] = sla.solve_triangular(D[i], R[i].conj().T, lower=True).conj().T  # :235-243
+        D[i + 1] = D[i + 1] - L[i] @ L[i].conj().T  # :247-251
+        R[i + 1] = R[i + 1] - R[i] @ L[i].conj().T  # :254-257
+        T[:, :] = T - R[i] @ R[i].conj().T  # :261-264
+        D[0] = D[0] - buffer[i] @ buffer[i].conj().T  # :268-270
+        buffer[i + 1] = -buffer[i] @ L[i].conj().T  # :273-275
+        R[0] = R[0] - R[i] @ buffer[i].conj().T  # :279-281
+
+
+def pobtasi_permuted(D, L, R, T, buffer):
+    """Follows src/serinv/algs/pobtasi.py:175-244 (_pobtasi_permuted) and, for
+    a == 0, pobtsi.py:128-179 (_pobtsi_permuted)."""
+    n, b = D.shape[0], D.shape[1]
+    eye_b = np.eye(b, dtype=D.dtype)
+    for i in range(n - 2, 0, -1):  # pobtasi.py:199
+        Li = L[i].copy()  # :200
+        Ri = R[i].copy()  # :201
+        Bi = buffer[i].copy()  # :202
+        Linv = sla.solve_triangular(D[i], eye_b, lower=True)  # :204-208
+        L[i] = (-buffer[i + 1].conj().T @ Bi - D[i + 1] @ Li - R[i + 1].conj().T @ Ri) @ Linv  # :211-217
+        buffer[i] = (-buffer[i + 1] @ Li - D[0] @ Bi - R[0].conj().T @ Ri) @ Linv  # :220-224
+        R[i] = (-R[i + 1] @ Li - R[0] @ Bi - T @ Ri) @ Linv  # :228-232
+        D[i] = (Linv.conj().T - L[i].conj().T @ Li - buffer[i].conj().T @ Bi
+                - R[i].conj().T @ Ri) @ Linv  # :235-241
+    L[0] = buffer[1].conj().T  # :244
+
+
+def pobtas_permuted(D, L, R, T, B, buffer, trans):
+    """Follows src/serinv/algs/pobtas.py:164-218 (_pobtas_permuted) and, for
+    a == 0, pobts.py:121-165 (_pobts_permuted)."""
+    n, b, a = D.shape[0], D.shape[1], R.shape[1]
+    blk = lambda i: slice(i * b, (i + 1) * b)  # noqa: E731
+    tip = slice(n * b, n * b + a)
+    if trans == "N":  # pobtas.py:179-204
+        for i in range(1, n - 1):
+            B[blk(i)] = sla.solve_triangular(D[i], B[blk(i)], lower=True)
+            B[blk(i + 1)] -= L[i] @ B[blk(i)]
+            B[blk(0)] -= buffer[i] @ B[blk(i)]
+            B[tip] -= R[i] @ B[blk(i)]
+    elif trans == "T" or trans == "C":  # :205-216
+        for i in range(n - 2, 0, -1):
+            B[blk(i)] = sla.solve_triangular(
+                D[i],
+                B[blk(i)] - L[i].conj().T @ B[blk(i + 1)] - R[i].conj().T @ B[tip]
+                - buffer[i].conj().T @ B[blk(0)],
+                lower=True,
+                trans="C",
+            )
+    else:
+        raise ValueError(f"Invalid transpose argument: {trans}.")  # :217-218
+
+
+# --------------------------------------------------------------------------
 # BT (arrowless): pobtf / pobtsi / pobts
 # --------------------------------------------------------------------------
+def _no_arrow(D):
+    return np.zeros((D.shape[0], 0, D.shape[1]), dtype=D.dtype)
+
+
 def pobtf(D, L, **kwargs):
     """Block Cholesky of an SPD BT matrix (src/serinv/algs/pobtf.py:11-122)."""
     _check_kwargs(kwargs, {"device_streaming", "buffer", "factorize_last_block"})
     _reject_unsupported(kwargs)
+    if kwargs.get("buffer", None) is not None:  # pobtf.py:53-68
+        return pobtaf_permuted(D, L, _no_arrow(D), np.zeros((0, 0)), kwargs["buffer"])
     factorize_last_block = kwargs.get("factorize_last_block", True)
     n = D.shape[0]
     for i in range(n - 1):  # pobtf.py:97
@@ -165,6 +243,8 @@ def pobtsi(D, L, **kwargs):
     """Selected inversion of an SPD BT matrix (src/serinv/algs/pobtsi.py:10-125)."""
     _check_kwargs(kwargs, {"device_streaming", "buffer", "invert_last_block"})
     _reject_unsupported(kwargs)
+    if kwargs.get("buffer", None) is not None:  # pobtsi.py:48-62
+        return pobtasi_permuted(D, L, _no_arrow(D), np.zeros((0, 0)), kwargs["buffer"])
     invert_last_block = kwargs.get("invert_last_block", True)
     n, b = D.shape[0], D.shape[1]
     eye_b = np.eye(b, dtype=D.dtype)  # pobtsi.py:92
@@ -182,7 +262,9 @@ def pobts(D, L, B, trans="N", **kwargs):
     """Forward/backward BT triangular solve (src/serinv/algs/pobts.py:10-118)."""
     partial_ = kwargs.get("partial", False)
     if kwargs.get("buffer", None) is not None:
-        raise NotImplementedError("permuted (buffer=...) variant is not restated")
+        if kwargs.get("device_streaming", False):  # pobts.py:29-40
+            raise NotImplementedError("Permuted arrowhead is not implemented.")
+        return pobtas_permuted(D, L, _no_arrow(D), np.zeros((0, 0)), B, kwargs["buffer"], trans)
     if kwargs.get("device_streaming", False):
         raise NotImplementedError("Streaming is not implemented for the natural arrowhead.")
     n, b = D.shape[0], D.shape[1]

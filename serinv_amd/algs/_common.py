@@ -66,10 +66,29 @@ def check_kwargs(kwargs, expected):
 
 
 def reject_unsupported(kwargs, what):
-    if kwargs.get("buffer", None) is not None:
-        raise NotImplementedError(f"{what}: the permuted (buffer=...) variant is not implemented")
     if kwargs.get("device_streaming", False):
         raise NotImplementedError(f"{what}: host-device streaming is not implemented")
+
+
+def permutation_buffer(kwargs, D, min_blocks=1):
+    """The buffer= argument of the permuted variants, validated: a tensor
+    like the diagonal blocks (serinv.utils.allocate_pobtax_permutation_buffers
+    returns empty_like(A_diagonal_blocks)), or None for the natural ordering.
+    The reference indexes buffer[1] (and L[0]), which fails for fewer than
+    min_blocks diagonal blocks with an IndexError; so does this."""
+    buf = kwargs.get("buffer", None)
+    if buf is None:
+        return None
+    _tensor(buf, "buffer")
+    if tuple(buf.shape) != tuple(D.shape):
+        raise ValueError(f"buffer must have the shape of the diagonal blocks {tuple(D.shape)}, "
+                         f"got {tuple(buf.shape)}")
+    if buf.device != D.device:
+        raise ValueError("buffer must be on the same device as the blocks")
+    if D.shape[0] < min_blocks:
+        raise IndexError(f"the permuted variant needs at least {min_blocks} diagonal blocks, "
+                         f"got {D.shape[0]}")
+    return buf
 
 
 def _tensor(x, name):

@@ -1,15 +1,16 @@
 """Block Cholesky factorization of an SPD BTA matrix (drop-in for serinv.algs.pobtaf).
 
-Reference: src/serinv/algs/pobtaf.py:11-94 (signature, kwargs, dispatch) and
-:97-183 (natural-ordering algorithm).  The factorization runs on the GPU as
-hand-written gfx950 HIP kernels (librocbta.so, rocbta_dpobtaf).
+Reference: src/serinv/algs/pobtaf.py:11-94 (signature, kwargs, dispatch),
+:97-183 (natural-ordering algorithm) and :186-281 (permuted algorithm, with
+buffer=).  The factorization runs on the GPU as hand-written gfx950 HIP
+kernels (librocbta.so, rocbta_dpobtaf / rocbta_dpobtaf_permuted).
 """
 
 import torch
 
 from .. import _lib
-from ._common import (bta_shapes, check_kwargs, config, note_status, ptr, raise_on_info,
-                      reject_unsupported, stream_of, workspace)
+from ._common import (bta_shapes, check_kwargs, config, note_status, permutation_buffer, ptr,
+                      raise_on_info, reject_unsupported, stream_of, workspace)
 
 
 def pobtaf(A_diagonal_blocks, A_lower_diagonal_blocks, A_lower_arrow_blocks, A_arrow_tip_block,
@@ -21,8 +22,12 @@ def pobtaf(A_diagonal_blocks, A_lower_diagonal_blocks, A_lower_arrow_blocks, A_a
     tensors on a ROCm device.  On exit they hold L_ii (upper triangle zero),
     L_{i+1,i}, L_{t,i} and L_T.
 
-    Keyword arguments: factorize_last_block (default True); device_streaming
-    and buffer are accepted names but only their defaults are supported.
+    Keyword arguments: factorize_last_block (default True); buffer (a
+    tensor like A_diagonal_blocks, see
+    serinv_amd.utils.allocate_pobtax_permutation_buffers) selects the
+    permuted variant, which factors the middle blocks 1 .. n-2 and leaves the
+    reduced system (blocks 0 and n-1, buffer[n-1], the tip) to the caller;
+    device_streaming is accepted but only its default is supported.
     A non-positive pivot raises numpy.linalg.LinAlgError (when
     serinv_amd.algs.config["check_info"] is True).
     """
@@ -31,6 +36,10 @@ def pobtaf(A_diagonal_blocks, A_lower_diagonal_blocks, A_lower_arrow_blocks, A_a
     factorize_last_block = bool(kwargs.get("factorize_last_block", True))
     D, L, R, T = A_diagonal_blocks, A_lower_diagonal_blocks, A_lower_arrow_blocks, A_arrow_tip_block
     n, b, a, dev = bta_shapes(D, L, R, T)
+    buf = permutation_buffer(kwargs, D, min_blocks=2)
+    if buf is not None:
+        _factor_permuted(D, L, R, T, buf, n, b, a, dev, "pobtaf")
+        return
     _factor(D, L, R, T, n, b, a, dev, factorize_last_block, "pobtaf")
 
 
@@ -41,6 +50,19 @@ def _factor(D, L, R, T, n, b, a, dev, factorize_last_block, what, mixed=False):
     fn = lib.rocbta_mpobtaf if mixed else lib.rocbta_dpobtaf
     rc = fn(n, b, a, ptr(D), ptr(L), ptr(R), ptr(T), int(factorize_last_block),
             info.data_ptr(), ws.data_ptr(), nbytes, stream_of(dev))
+    _lib.check(rc, what)
+    if config["check_info"]:
+        raise_on_info(info, what)
+    else:
+        note_status(info, what)
+
+
+def _factor_permuted(D, L, R, T, buf, n, b, a, dev, what):
+    lib = _lib.load()
+    ws, nbytes = workspace(_lib.OP_POBTAF_PERMUTED, n, b, a, 0, dev)
+    info = torch.empty(1, dtype=torch.int32, device=dev)
+    rc = lib.rocbta_dpobtaf_permuted(n, b, a, ptr(D), ptr(L), ptr(R), ptr(T), ptr(buf),
+                                     info.data_ptr(), ws.data_ptr(), nbytes, stream_of(dev))
     _lib.check(rc, what)
     if config["check_info"]:
         raise_on_info(info, what)

@@ -1,14 +1,16 @@
 """Triangular solve with the BTA block Cholesky factor (drop-in for serinv.algs.pobtas).
 
-Reference: src/serinv/algs/pobtas.py:10-62 (signature, kwargs, dispatch) and
-:65-161 (algorithm); runs as rocbta_dpobtas.  Like the reference, pobtas does
+Reference: src/serinv/algs/pobtas.py:10-62 (signature, kwargs, dispatch),
+:65-161 (algorithm) and :164-218 (permuted algorithm, with buffer=); runs as
+rocbta_dpobtas / rocbta_dpobtas_permuted.  Like the reference, pobtas does
 not validate its keyword arguments.
 """
 
 import torch
 
 from .. import _lib
-from ._common import _tensor, bta_shapes, config, note_status, ptr, stream_of, workspace
+from ._common import (_tensor, bta_shapes, config, note_status, permutation_buffer, ptr, stream_of,
+                      workspace)
 
 
 def pobtas(L_diagonal_blocks, L_lower_diagonal_blocks, L_lower_arrow_blocks, L_arrow_tip_block, B,
@@ -17,9 +19,11 @@ def pobtas(L_diagonal_blocks, L_lower_diagonal_blocks, L_lower_arrow_blocks, L_a
 
     trans "N": B <- L^{-1} B (forward); "T" or "C": B <- L^{-T} B (backward).
     Keyword arguments: partial (default False) skips the last-block and tip
-    solves; device_streaming / buffer raise NotImplementedError as in the
-    reference (pobtas.py:33-36, 49-52) or because the permuted variant is not
-    implemented.
+    solves; buffer (from pobtaf(..., buffer=)) selects the permuted variant,
+    whose forward pass eliminates the middle blocks 1 .. n-2 into the rows of
+    block 0, block n-1 and the tip, and whose backward pass takes the reduced
+    system's solution from those rows; device_streaming raises
+    NotImplementedError as in the reference (pobtas.py:33-36, 49-52).
     """
     D, L, R, T = L_diagonal_blocks, L_lower_diagonal_blocks, L_lower_arrow_blocks, L_arrow_tip_block
     n, b, a, dev = bta_shapes(D, L, R, T)
@@ -28,11 +32,15 @@ def pobtas(L_diagonal_blocks, L_lower_diagonal_blocks, L_lower_arrow_blocks, L_a
 
 def _solve(D, L, R, T, B, trans, kwargs, n, b, a, dev, what):
     partial = bool(kwargs.get("partial", False))
+    streaming = kwargs.get("device_streaming", False)
+    buf = None
     if kwargs.get("buffer", None) is not None:
-        if kwargs.get("device_streaming", False):
-            raise NotImplementedError("Streaming is not implemented for the permuted arrowhead.")
-        raise NotImplementedError(f"{what}: the permuted (buffer=...) variant is not implemented")
-    if kwargs.get("device_streaming", False):
+        if streaming:  # pobtas.py:31-36, pobts.py:29-32
+            raise NotImplementedError("Streaming is not implemented for the permuted arrowhead."
+                                      if what == "pobtas" else
+                                      "Permuted arrowhead is not implemented.")
+        buf = permutation_buffer(kwargs, D)
+    elif streaming:
         raise NotImplementedError("Streaming is not implemented for the natural arrowhead.")
     if trans not in ("N", "T", "C"):
         raise ValueError(f"Invalid transpose argument: {trans}.")
@@ -49,6 +57,13 @@ def _solve(D, L, R, T, B, trans, kwargs, n, b, a, dev, what):
     if B.device != dev:
         raise ValueError("B must be on the same device as the factor")
     lib = _lib.load()
+    if buf is not None:  # pobtas.py:164-218 / pobts.py:121-165 (partial does not apply)
+        ws, nbytes = workspace(_lib.OP_POBTAS_PERMUTED, n, b, a, r, dev)
+        rc = lib.rocbta_dpobtas_permuted(n, b, a, ptr(D), ptr(L), ptr(R), ptr(T), ptr(buf), ptr(B),
+                                         r, ldb, trans.encode(), ws.data_ptr(), nbytes,
+                                         stream_of(dev))
+        _lib.check(rc, what)
+        return
     ws, nbytes = workspace(_lib.OP_POBTAS, n, b, a, r, dev)
     rc = lib.rocbta_dpobtas(n, b, a, ptr(D), ptr(L), ptr(R), ptr(T), ptr(B), r, ldb,
                             trans.encode(), int(partial), ws.data_ptr(), nbytes, stream_of(dev))

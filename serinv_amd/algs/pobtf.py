@@ -1,11 +1,12 @@
 """Block Cholesky factorization of an SPD BT matrix (drop-in for serinv.algs.pobtf).
 
-Reference: src/serinv/algs/pobtf.py:11-80 (signature, kwargs, dispatch) and
-:83-122 (algorithm); runs as rocbta_dpobtaf with a = 0.
+Reference: src/serinv/algs/pobtf.py:11-80 (signature, kwargs, dispatch),
+:83-122 (algorithm) and :125-185 (permuted algorithm, with buffer=); runs as
+rocbta_dpobtaf / rocbta_dpobtaf_permuted with a = 0.
 """
 
-from ._common import bta_shapes, check_kwargs, reject_unsupported
-from .pobtaf import _factor
+from ._common import bta_shapes, check_kwargs, permutation_buffer, reject_unsupported
+from .pobtaf import _factor, _factor_permuted
 
 
 def pobtf(A_diagonal_blocks, A_lower_diagonal_blocks, **kwargs):
@@ -19,4 +20,8 @@ def pobtf(A_diagonal_blocks, A_lower_diagonal_blocks, **kwargs):
     factorize_last_block = bool(kwargs.get("factorize_last_block", True))
     D, L = A_diagonal_blocks, A_lower_diagonal_blocks
     n, b, a, dev = bta_shapes(D, L, None, None)
+    buf = permutation_buffer(kwargs, D, min_blocks=2)
+    if buf is not None:  # pobtf.py:53-68, 125-185
+        _factor_permuted(D, L, None, None, buf, n, b, 0, dev, "pobtf")
+        return
     _factor(D, L, None, None, n, b, 0, dev, factorize_last_block, "pobtf")

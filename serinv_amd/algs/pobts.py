@@ -1,7 +1,8 @@
 """Triangular solve with the BT block Cholesky factor (drop-in for serinv.algs.pobts).
 
-Reference: src/serinv/algs/pobts.py:10-54 (signature, kwargs, dispatch) and
-:57-118 (algorithm); runs as rocbta_dpobtas with a = 0.
+Reference: src/serinv/algs/pobts.py:10-54 (signature, kwargs, dispatch),
+:57-118 (algorithm) and :121-165 (permuted algorithm, with buffer=); runs as
+rocbta_dpobtas / rocbta_dpobtas_permuted with a = 0.
 """
 
 from ._common import bta_shapes

@@ -49,6 +49,7 @@ struct GemmList {
   std::vector<GemmProblem> probs;
   GemmList() { probs.reserve(8); }
 
+  GemmProblem& problems_back() { return probs.back(); }
   GemmProblem& add(double* C, int64_t ldc, int M, int N, double beta, int uplo = TRI_NONE,
                    int batch1 = 1, int64_t sC1 = 0, int batch2 = 1, int64_t sC2 = 0) {
     GemmProblem p{};
@@ -929,6 +930,44 @@ int pobtas_block(int64_t n, int64_t b, int64_t a, const double* D, const double*
   return ROCBTA_OK;
 }
 
+// Workspace of the permuted selected inversion: the inverses of the diagonal
+// blocks (all n, by the batched trtri), its scratch, copies of the factor
+// blocks a step overwrites while still reading them (L_{i+1,i}, L_{top,i},
+// L_{t,i}), and the bracketed sums of the step's four products.
+struct PsiLayout {
+  size_t linv, w, lt, bt, rt, wl, wb, wr, wd, total;
+};
+PsiLayout psi_layout(int64_t n, int64_t b, int64_t a) {
+  PsiLayout l{};
+  size_t off = 0;
+  auto take = [&](size_t bytes) {
+    size_t o = off;
+    off = align256(off + bytes);
+    return o;
+  };
+  const int64_t ntb = (b + TT - 1) / TT;
+  const size_t bb = (size_t)b * b * sizeof(double), ab = (size_t)std::max<int64_t>(a, 1) * b * sizeof(double);
+  l.linv = take((size_t)n * bb);
+  l.w = take((size_t)n * std::max<int64_t>(ntb * TT * TT, ntb * TT * ntb * TT / 4) * sizeof(double));
+  l.lt = take(bb);
+  l.bt = take(bb);
+  l.rt = take(ab);
+  l.wl = take(bb);
+  l.wb = take(bb);
+  l.wr = take(ab);
+  l.wd = take(bb);
+  l.total = off;
+  return l;
+}
+
+// dst = src^T for one b x b block (64 x 64 tiles through LDS).
+int transpose_block(const double* src, double* dst, int64_t b, hipStream_t s) {
+  const int T = (int)((b + 63) / 64);
+  transpose_blocks_kernel<<<dim3(T * T, 1), 256, 0, s>>>(src, dst, (int)b, b * b, T);
+  RB_CHECK(hipGetLastError());
+  return ROCBTA_OK;
+}
+
 int logdet_parts(int64_t n, int64_t b, int64_t a) {
   return (int)((n * b + a + LD_ROWS_PER_WG - 1) / LD_ROWS_PER_WG);
 }
@@ -962,6 +1001,12 @@ size_t rocbta_workspace_bytes(int op, int64_t n, int64_t b, int64_t a, int64_t r
     }
     case ROCBTA_OP_LOGDET:
       return align256((size_t)logdet_parts(n, b, a) * sizeof(double));
+    case ROCBTA_OP_POBTAF_PERMUTED:
+      return align256((size_t)TT * TT * sizeof(double));
+    case ROCBTA_OP_POBTASI_PERMUTED:
+      return psi_layout(n, b, a).total;
+    case ROCBTA_OP_POBTAS_PERMUTED:
+      return 256;
     case ROCBTA_OP_RESIDUAL: {
       const int64_t rr = std::max<int64_t>(r, 1), rc = sweep_rc(rr);
       return align256((size_t)cdiv(rr, rc) * n * cdiv(a, SW_T) * SW_T * rc * sizeof(double)) + 256;
@@ -1561,6 +1606,202 @@ int rocbta_dlogdet(int64_t n, int64_t b, int64_t a, const double* D, const doubl
   RB_CHECK(hipGetLastError());
   logdet_finish_kernel<<<1, 256, 0, s>>>(part, nparts, out);
   RB_CHECK(hipGetLastError());
+  return ROCBTA_OK;
+}
+
+// -------------------------------------------------------------- permuted
+// The "middle partition" forms of the distributed algorithm: the first and
+// last diagonal blocks and the tip are left to a reduced system, and a top
+// buffer column (buffer[i] = the coupling of block i with block 0) carries
+// the fill-in.  Block recurrences over the same primitives as the natural
+// forms: the blocked diagonal factorization with its panel solves
+// (factor_block_column), the batched block inverse (block_trtri), grouped
+// GEMMs and the 64-row tile solves.  a = 0 gives the BT forms.
+
+int rocbta_dpobtaf_permuted(int64_t n, int64_t b, int64_t a, double* D, double* L, double* R,
+                            double* T, double* buffer, int* info, void* workspace,
+                            size_t workspace_bytes, void* stream) {
+  if (bad_shape(n, b, a) || !D || !buffer || !info || (n > 1 && !L) || (a > 0 && (!R || !T)))
+    return ROCBTA_ERR_ARG;
+  if (!workspace || workspace_bytes < rocbta_workspace_bytes(ROCBTA_OP_POBTAF_PERMUTED, n, b, a, 0))
+    return ROCBTA_ERR_WORKSPACE;
+  hipStream_t s = (hipStream_t)stream;
+  set_int_kernel<<<1, 64, 0, s>>>(info, 0);
+  RB_CHECK(hipGetLastError());
+  double* tinv = (double*)workspace;
+  const int bi = (int)b, ai = (int)a;
+  const int64_t bb = b * b, ab = a * b;
+  // buffer[1] = A_{1,0}^T, the top block's coupling with block 1 (pobtaf.py:204)
+  if (n >= 2) {
+    int rc = transpose_block(L, buffer + bb, b, s);
+    if (rc) return rc;
+  }
+  for (int64_t i = 1; i + 1 < n; ++i) {  // pobtaf.py:207-281
+    // L_ii = chol(A_ii); L_{i+1,i}, L_{top,i} = buffer[i], L_{t,i} = R_i
+    // times L_ii^{-T}
+    std::vector<Panel> panels = {{L + i * bb, bi, b}, {buffer + i * bb, bi, b}};
+    if (a > 0) panels.push_back({R + i * ab, ai, b});
+    int rc = factor_block_column(D + i * bb, b, bi, panels, i * b, info, tinv, s);
+    if (rc) return rc;
+    const double* Li = L + i * bb;
+    const double* Bi = buffer + i * bb;
+    const double* Ri = R + i * ab;
+    // Schur updates (independent outputs in one group).  Block i+1 is
+    // factored next unless it is the last one, which keeps the full
+    // symmetric update, like the top block D_0 and the tip.
+    GemmList upd;
+    const bool last = (i + 2 == n);
+    push_term(upd.add(D + (i + 1) * bb, b, bi, bi, 1.0, last ? TRI_NONE : TRI_LOWER),
+              term(Li, b, false, Li, b, true, bi, -1.0));
+    push_term(upd.add(D, b, bi, bi, 1.0), term(Bi, b, false, Bi, b, true, bi, -1.0));
+    push_term(upd.add(buffer + (i + 1) * bb, b, bi, bi, 0.0), term(Bi, b, false, Li, b, true, bi, -1.0));
+    if (a > 0) {
+      push_term(upd.add(R + (i + 1) * ab, b, ai, bi, 1.0), term(Ri, b, false, Li, b, true, bi, -1.0));
+      push_term(upd.add(T, a, ai, ai, 1.0), term(Ri, b, false, Ri, b, true, bi, -1.0));
+      push_term(upd.add(R, b, ai, bi, 1.0), term(Ri, b, false, Bi, b, true, bi, -1.0));
+    }
+    rc = run_gemms(upd, s);
+    if (rc) return rc;
+  }
+  return ROCBTA_OK;
+}
+
+int rocbta_dpobtasi_permuted(int64_t n, int64_t b, int64_t a, double* D, double* L, double* R,
+                             double* T, double* buffer, void* workspace, size_t workspace_bytes,
+                             void* stream) {
+  if (bad_shape(n, b, a) || !D || !buffer || (n > 1 && !L) || (a > 0 && (!R || !T)))
+    return ROCBTA_ERR_ARG;
+  const PsiLayout lay = psi_layout(n, b, a);
+  if (!workspace || workspace_bytes < lay.total) return ROCBTA_ERR_WORKSPACE;
+  hipStream_t s = (hipStream_t)stream;
+  char* ws = (char*)workspace;
+  double* Linv = (double*)(ws + lay.linv);
+  double* Lt = (double*)(ws + lay.lt);
+  double* Bt = (double*)(ws + lay.bt);
+  double* Rt = (double*)(ws + lay.rt);
+  double* WL = (double*)(ws + lay.wl);
+  double* WB = (double*)(ws + lay.wb);
+  double* WR = (double*)(ws + lay.wr);
+  double* WD = (double*)(ws + lay.wd);
+  const int bi = (int)b, ai = (int)a;
+  const int64_t bb = b * b, ab = a * b;
+  if (n >= 3) {
+    // L_ii^{-1} of the middle blocks (pobtasi.py:205-209), batched
+    int rc = block_trtri(D + bb, b, bb, Linv + bb, b, bb, bi, (int)(n - 2), (double*)(ws + lay.w), s,
+                         true);
+    if (rc) return rc;
+  }
+  for (int64_t i = n - 2; i >= 1; --i) {  // pobtasi.py:199-240
+    const double* Li_inv = Linv + i * bb;
+    // the factor blocks this step overwrites but still reads
+    RB_CHECK(hipMemcpyAsync(Lt, L + i * bb, bb * sizeof(double), hipMemcpyDeviceToDevice, s));
+    RB_CHECK(hipMemcpyAsync(Bt, buffer + i * bb, bb * sizeof(double), hipMemcpyDeviceToDevice, s));
+    if (a > 0)
+      RB_CHECK(hipMemcpyAsync(Rt, R + i * ab, ab * sizeof(double), hipMemcpyDeviceToDevice, s));
+    const double* X1 = D + (i + 1) * bb;      // X_{i+1,i+1}
+    const double* B1 = buffer + (i + 1) * bb;  // X_{top,i+1}
+    const double* R1 = R + (i + 1) * ab;       // X_{t,i+1}
+    // bracketed sums of X_{i+1,i}, X_{top,i}, X_{t,i}
+    GemmList w1;
+    push_term(w1.add(WL, b, bi, bi, 0.0), term(B1, b, true, Bt, b, false, bi, -1.0));
+    push_term(w1.problems_back(), term(X1, b, false, Lt, b, false, bi, -1.0));
+    push_term(w1.add(WB, b, bi, bi, 0.0), term(B1, b, false, Lt, b, false, bi, -1.0));
+    push_term(w1.problems_back(), term(D, b, false, Bt, b, false, bi, -1.0));
+    if (a > 0) {
+      push_term(w1.add(WR, b, ai, bi, 0.0), term(R1, b, false, Lt, b, false, bi, -1.0));
+      push_term(w1.problems_back(), term(R, b, false, Bt, b, false, bi, -1.0));
+    }
+    int rc = run_gemms(w1, s);
+    if (rc) return rc;
+    if (a > 0) {
+      GemmList w2;
+      push_term(w2.add(WL, b, bi, bi, 1.0), term(R1, b, true, Rt, b, false, ai, -1.0));
+      push_term(w2.add(WB, b, bi, bi, 1.0), term(R, b, true, Rt, b, false, ai, -1.0));
+      push_term(w2.add(WR, b, ai, bi, 1.0), term(T, a, false, Rt, b, false, ai, -1.0));
+      rc = run_gemms(w2, s);
+      if (rc) return rc;
+    }
+    // times L_ii^{-1} (lower triangular: K trimmed)
+    GemmList w3;
+    push_term(w3.add(L + i * bb, b, bi, bi, 0.0),
+              term(WL, b, false, Li_inv, b, false, bi, 1.0, TRI_NONE, TRI_LOWER));
+    push_term(w3.add(buffer + i * bb, b, bi, bi, 0.0),
+              term(WB, b, false, Li_inv, b, false, bi, 1.0, TRI_NONE, TRI_LOWER));
+    if (a > 0)
+      push_term(w3.add(R + i * ab, b, ai, bi, 0.0),
+                term(WR, b, false, Li_inv, b, false, bi, 1.0, TRI_NONE, TRI_LOWER));
+    rc = run_gemms(w3, s);
+    if (rc) return rc;
+    // X_ii = (L_ii^{-T} - X_{i+1,i}^T L_{i+1,i} - X_{top,i}^T L_{top,i} - X_{t,i}^T L_{t,i}) L_ii^{-1}
+    GemmList w4;
+    push_term(w4.add(WD, b, bi, bi, 0.0), term(L + i * bb, b, true, Lt, b, false, bi, -1.0));
+    push_term(w4.problems_back(), term(buffer + i * bb, b, true, Bt, b, false, bi, -1.0));
+    rc = run_gemms(w4, s);
+    if (rc) return rc;
+    if (a > 0) {
+      GemmList w5;
+      push_term(w5.add(WD, b, bi, bi, 1.0), term(R + i * ab, b, true, Rt, b, false, ai, -1.0));
+      rc = run_gemms(w5, s);
+      if (rc) return rc;
+    }
+    GemmList w6;
+    push_term(w6.add(D + i * bb, b, bi, bi, 0.0),
+              term(WD, b, false, Li_inv, b, false, bi, 1.0, TRI_NONE, TRI_LOWER));
+    push_term(w6.problems_back(),
+              term(Li_inv, b, true, Li_inv, b, false, bi, 1.0, TRI_UPPER, TRI_LOWER));
+    rc = run_gemms(w6, s);
+    if (rc) return rc;
+  }
+  // X_{1,0} = X_{top,1}^T back to the tridiagonal storage (pobtasi.py:244)
+  if (n >= 2) return transpose_block(buffer + bb, L, b, s);
+  return ROCBTA_OK;
+}
+
+int rocbta_dpobtas_permuted(int64_t n, int64_t b, int64_t a, const double* D, const double* L,
+                            const double* R, const double* T, const double* buffer, double* B,
+                            int64_t r, int64_t ldb, char trans, void* workspace,
+                            size_t workspace_bytes, void* stream) {
+  (void)T;
+  if (bad_shape(n, b, a) || !D || !B || !buffer || (n > 1 && !L) || (a > 0 && !R) || r < 0 ||
+      ldb < r)
+    return ROCBTA_ERR_ARG;
+  if (trans != 'N' && trans != 'T' && trans != 'C') return ROCBTA_ERR_ARG;
+  if (r == 0) return ROCBTA_OK;
+  hipStream_t s = (hipStream_t)stream;
+  if (workspace && workspace_bytes >= sizeof(int)) {
+    set_int_kernel<<<1, 64, 0, s>>>((int*)workspace, 0);  // status word: no time-out
+    RB_CHECK(hipGetLastError());
+  }
+  const int bi = (int)b, ai = (int)a, ri = (int)r;
+  const int64_t bb = b * b, ab = a * b;
+  double* Bt = B + n * b * ldb;  // tip rows
+  if (trans == 'N') {  // pobtas.py:179-204
+    for (int64_t i = 1; i + 1 < n; ++i) {
+      std::vector<Below> below = {{L + i * bb, b, bi, B + (i + 1) * b * ldb},
+                                  {buffer + i * bb, b, bi, B}};
+      if (a > 0) below.push_back({R + i * ab, b, ai, Bt});
+      int rc = forward_block(D + i * bb, b, bi, B + i * b * ldb, ldb, ri, below, s);
+      if (rc) return rc;
+    }
+    return ROCBTA_OK;
+  }
+  for (int64_t i = n - 2; i >= 1; --i) {  // pobtas.py:205-216
+    double* Bi = B + i * b * ldb;
+    GemmList upd;
+    GemmProblem& p = upd.add(Bi, ldb, bi, ri, 1.0);
+    push_term(p, term(L + i * bb, b, true, B + (i + 1) * b * ldb, ldb, false, bi, -1.0));
+    push_term(p, term(buffer + i * bb, b, true, B, ldb, false, bi, -1.0));
+    int rc = run_gemms(upd, s);
+    if (rc) return rc;
+    if (a > 0) {
+      GemmList upd2;
+      push_term(upd2.add(Bi, ldb, bi, ri, 1.0), term(R + i * ab, b, true, Bt, ldb, false, ai, -1.0));
+      rc = run_gemms(upd2, s);
+      if (rc) return rc;
+    }
+    rc = backward_block(D + i * bb, b, bi, Bi, ldb, ri, s);
+    if (rc) return rc;
+  }
   return ROCBTA_OK;
 }
 

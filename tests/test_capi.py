@@ -82,3 +82,30 @@ def test_batched_launch_shape_bits_checked(lib):
     ops = _lib.BATCHED_FACTOR | _lib.BATCHED_SOLVE | _lib.BATCHED_RC[1] | _lib.BATCHED_RC[4]
     rc = lib.rocbta_batched_create(1, n, b, a, fake, fake, fake, fake, fake, r, ops, ctypes.byref(plan))
     assert rc == _lib.ROCBTA_ERR_ARG and not plan.value
+
+
+def test_permuted_workspace_and_argument_errors(lib):
+    n, b, a = 16, 256, 64
+    assert lib.rocbta_workspace_bytes(_lib.OP_POBTAF_PERMUTED, n, b, a, 0) >= 64 * 64 * 8
+    # the diagonal blocks' inverses plus the per-step copies and sums
+    assert lib.rocbta_workspace_bytes(_lib.OP_POBTASI_PERMUTED, n, b, a, 0) >= (n + 6) * b * b * 8
+    assert lib.rocbta_workspace_bytes(_lib.OP_POBTAS_PERMUTED, n, b, a, 4) >= 4
+    assert lib.rocbta_workspace_bytes(_lib.OP_POBTASI_PERMUTED, 0, b, a, 0) == 0
+    fake = ctypes.c_void_p(16)
+    # missing buffer / info / workspace, bad trans: rejected on the host
+    assert lib.rocbta_dpobtaf_permuted(3, 4, 1, fake, fake, fake, fake, None, fake, fake, 1 << 20,
+                                       None) == _lib.ROCBTA_ERR_ARG
+    assert lib.rocbta_dpobtaf_permuted(3, 4, 1, fake, fake, fake, fake, fake, None, fake, 1 << 20,
+                                       None) == _lib.ROCBTA_ERR_ARG
+    assert lib.rocbta_dpobtaf_permuted(3, 4, 1, fake, fake, fake, fake, fake, fake, None, 0,
+                                       None) == _lib.ROCBTA_ERR_WORKSPACE
+    assert lib.rocbta_dpobtasi_permuted(3, 4, 1, fake, fake, fake, fake, None, fake, 1 << 20,
+                                        None) == _lib.ROCBTA_ERR_ARG
+    assert lib.rocbta_dpobtasi_permuted(3, 4, 1, fake, fake, fake, fake, fake, fake, 16,
+                                        None) == _lib.ROCBTA_ERR_WORKSPACE
+    assert lib.rocbta_dpobtas_permuted(3, 4, 1, fake, fake, fake, fake, fake, fake, 1, 1, b"X",
+                                       None, 0, None) == _lib.ROCBTA_ERR_ARG
+    assert lib.rocbta_dpobtas_permuted(3, 4, 1, fake, fake, fake, fake, None, fake, 1, 1, b"N",
+                                       None, 0, None) == _lib.ROCBTA_ERR_ARG
+    assert lib.rocbta_dpobtas_permuted(3, 4, 1, fake, fake, fake, fake, fake, fake, 2, 1, b"N",
+                                       None, 0, None) == _lib.ROCBTA_ERR_ARG  # ldb < r

@@ -115,3 +115,102 @@ def test_logdet_mutant_is_caught():
     bad = mutant(serinv_np.logdet_from_factor, "s += 2.0 * np.sum(np.log(np.diag(T)))", "pass")
     assert abs(bad(D, T) - ld) > 1e-6
     assert abs(serinv_np.logdet_from_factor(D, T) - ld) < 1e-10
+
+
+# Permuted forms: the mutant replaces one routine in the reduced-system
+# procedure, checked against the dense inverse / solve, and the mutated
+# factorization also against the Cholesky of the block-permuted matrix.
+from tests._permuted import dense_permuted_factor, selinv_permuted, solve_permuted  # noqa: E402
+
+
+def permuted_rt(f=None, si=None, sv=None):
+    class RT:
+        @staticmethod
+        def pobtaf(D, L, R, T, buffer=None):
+            if buffer is None:
+                serinv_np.pobtaf(D, L, R, T)
+            else:
+                (f or serinv_np.pobtaf_permuted)(D, L, R, T, buffer)
+
+        @staticmethod
+        def pobtasi(D, L, R, T, buffer=None):
+            if buffer is None:
+                serinv_np.pobtasi(D, L, R, T)
+            else:
+                (si or serinv_np.pobtasi_permuted)(D, L, R, T, buffer)
+
+        @staticmethod
+        def pobtas(D, L, R, T, B, buffer=None, trans="N"):
+            if buffer is None:
+                serinv_np.pobtas(D, L, R, T, B, trans=trans)
+            else:
+                (sv or serinv_np.pobtas_permuted)(D, L, R, T, B, buffer, trans)
+    return RT
+
+
+def permuted_errs(rt, n=5):
+    A, b, a, n = case(n=n)
+    D, L, R, T = dense.bta_dense_to_arrays(A, b, a, n)
+    buf = np.zeros_like(D)
+    rt.pobtaf(D, L, R, T, buffer=buf)
+    ref = dense_permuted_factor(A, n, b, a)
+    fac = max(np.max(np.abs(x - ref[k])) for k, x in zip("DLRT", (D, L, R, T)))
+    fac = max(fac, np.max(np.abs(buf[1:] - ref["buffer"][1:])))
+    D, L, R, T = dense.bta_dense_to_arrays(A, b, a, n)
+    buf = np.zeros_like(D)
+    selinv_permuted(D, L, R, T, buf, rt, np.zeros)
+    ref = dense.bta_dense_to_arrays(np.linalg.inv(A), b, a, n)
+    si = max(np.max(np.abs(x - y)) / np.max(np.abs(y)) for x, y in zip((D, L, R, T), ref))
+    Bm = np.random.default_rng(5).random((A.shape[0], 2))
+    D, L, R, T = dense.bta_dense_to_arrays(A, b, a, n)
+    buf = np.zeros_like(D)
+    B = Bm.copy()
+    solve_permuted(D, L, R, T, buf, B, rt, np.zeros)
+    sv = np.max(np.abs(B - np.linalg.solve(A, Bm)))
+    return fac, si, sv
+
+
+def test_unmutated_permuted_oracle_passes():
+    assert max(permuted_errs(permuted_rt())) < TOL
+
+
+PERMUTED_FACTOR_MUTANTS = [
+    ("D[0] = D[0] - buffer[i] @ buffer[i].conj().T", "D[0] = D[0] + buffer[i] @ buffer[i].conj().T"),
+    ("buffer[i + 1] = -buffer[i] @ L[i].conj().T", "buffer[i + 1] = buffer[i] @ L[i].conj().T"),
+    ("R[0] = R[0] - R[i] @ buffer[i].conj().T", "R[0] = R[0] - R[i] @ L[i].conj().T"),
+    ("buffer[1] = L[0].conj().T", "buffer[1] = L[1].conj().T"),  # wrong index
+]
+
+
+@pytest.mark.parametrize("old,new", PERMUTED_FACTOR_MUTANTS)
+def test_permuted_factor_mutants_are_caught(old, new):
+    bad = mutant(serinv_np.pobtaf_permuted, old, new)
+    fac, si, sv = permuted_errs(permuted_rt(f=bad))
+    assert fac > 1e-6 and si > 1e-6 and sv > 1e-6
+
+
+PERMUTED_SELINV_MUTANTS = [
+    (" - R[0].conj().T @ Ri) @ Linv", ") @ Linv"),  # dropped arrow term
+    ("-buffer[i + 1].conj().T @ Bi", "-buffer[i + 1] @ Bi"),  # transposed operand
+    ("- D[0] @ Bi", "- D[i] @ Bi"),  # wrong index
+    ("L[0] = buffer[1].conj().T", "pass"),
+]
+
+
+@pytest.mark.parametrize("old,new", PERMUTED_SELINV_MUTANTS)
+def test_permuted_selinv_mutants_are_caught(old, new):
+    bad = mutant(serinv_np.pobtasi_permuted, old, new)
+    assert permuted_errs(permuted_rt(si=bad))[1] > 1e-6
+
+
+PERMUTED_SOLVE_MUTANTS = [
+    ("- buffer[i].conj().T @ B[blk(0)]", "+ buffer[i].conj().T @ B[blk(0)]"),
+    ("B[blk(0)] -= buffer[i] @ B[blk(i)]", "pass"),
+    ("B[tip] -= R[i] @ B[blk(i)]", "B[tip] -= R[i - 1] @ B[blk(i)]"),
+]
+
+
+@pytest.mark.parametrize("old,new", PERMUTED_SOLVE_MUTANTS)
+def test_permuted_solve_mutants_are_caught(old, new):
+    bad = mutant(serinv_np.pobtas_permuted, old, new)
+    assert permuted_errs(permuted_rt(sv=bad))[2] > 1e-6
