@@ -43,6 +43,7 @@ def parse():
     p.add_argument("--no-solve", action="store_true")
     p.add_argument("--no-batched", action="store_true")
     p.add_argument("--no-mixed", action="store_true")
+    p.add_argument("--no-permuted", action="store_true")
     return p.parse_args()
 
 
@@ -265,6 +266,64 @@ def bench_batched(dev, nmat=64, r=4, reps=3, tune=True, mixed=False):
         "info_ok": bool(np.all(info == 0) and status == 0),
         "autotune": tuned,
     }
+
+
+def bench_permuted(dev, n, b, a, reps=2):
+    """One partition's work in the distributed form: the permuted ("middle
+    partition") pobtaf and pobtasi on n blocks, completed through the 2-block
+    reduced system, against the natural pobtaf + pobtasi of the same matrix
+    (the first partition's form).  Both give the same selected inverse; the
+    largest relative difference over D, L, R, T is reported."""
+    import torch
+
+    import serinv_amd.algs as algs
+    from serinv_amd.utils import allocate_pobtax_permutation_buffers
+    from serinv_amd.utils.matrices import bta_spd
+
+    D0, L0, R0, T0, _ = (None if x is None else torch.from_numpy(x).to(dev)
+                         for x in bta_spd(n, b, a, seed=63))
+    stream = torch.cuda.current_stream(dev)
+
+    def timed(fn):
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        fn()
+        e1.record(stream)
+        torch.cuda.synchronize()
+        return e0.elapsed_time(e1)
+
+    res = {"n": n, "b": b, "a": a}
+    nat_af, nat_si, per_af, per_si = [], [], [], []
+    for _ in range(reps):
+        D, L, R, T = (x.clone() for x in (D0, L0, R0, T0))
+        nat_af.append(timed(lambda: algs.pobtaf(D, L, R, T)))
+        nat_si.append(timed(lambda: algs.pobtasi(D, L, R, T)))
+        Dn, Ln, Rn, Tn = D, L, R, T
+        D, L, R, T = (x.clone() for x in (D0, L0, R0, T0))
+        buf = allocate_pobtax_permutation_buffers(D)
+        per_af.append(timed(lambda: algs.pobtaf(D, L, R, T, buffer=buf)))
+        # reduced system (block 0, block n-1, tip): natural factor + inverse
+        rD = torch.stack([D[0], D[-1]]).contiguous()
+        rL = buf[-1].T.contiguous().unsqueeze(0)
+        rR = torch.stack([R[0], R[-1]]).contiguous()
+        rT = T.clone()
+        algs.pobtaf(rD, rL, rR, rT)
+        algs.pobtasi(rD, rL, rR, rT)
+        D[0], D[-1], buf[-1] = rD[0], rD[1], rL[0].T
+        R[0], R[-1] = rR[0], rR[1]
+        T.copy_(rT)
+        per_si.append(timed(lambda: algs.pobtasi(D, L, R, T, buffer=buf)))
+    torch.cuda.synchronize()
+
+    def rel(x, y):
+        return float((x - y).abs().max() / y.abs().max()) if y.numel() else 0.0
+
+    res["natural_pobtaf_ms"] = round(min(nat_af), 3)
+    res["natural_pobtasi_ms"] = round(min(nat_si), 3)
+    res["permuted_pobtaf_ms"] = round(min(per_af), 3)
+    res["permuted_pobtasi_ms"] = round(min(per_si), 3)
+    res["max_rel_diff_vs_natural"] = float(f"{max(rel(D, Dn), rel(L, Ln), rel(R, Rn), rel(T, Tn)):.3e}")
+    return res
 
 
 def bench_mixed(dev, n, b, a, kappas=(1e4, 1e6, 1e8), r=16, reps=2):
@@ -512,6 +571,10 @@ def main():
             print(f"[bench] mixed kappa {m['kappa_target']:.0e}: {m['mixed_ms']} ms vs {m['fp64_ms']} ms, "
                   f"err X {max(m['err_X'].values()):.2e}, refined x {m['err_x_refined']:.2e}",
                   file=sys.stderr, flush=True)
+    if not args.no_permuted:
+        print("[bench] permuted (middle-partition) form, n=64 ...", file=sys.stderr, flush=True)
+        out["permuted"] = bench_permuted(dev, 64, b, a)
+        print(f"[bench] permuted {out['permuted']}", file=sys.stderr, flush=True)
     if not args.no_batched:
         print("[bench] batched plan vs per-matrix loop ...", file=sys.stderr, flush=True)
         out["batched"] = bench_batched(dev)
