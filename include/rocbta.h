@@ -43,6 +43,8 @@ extern "C" {
 #define ROCBTA_OP_POBTAF_PERMUTED 5
 #define ROCBTA_OP_POBTASI_PERMUTED 6
 #define ROCBTA_OP_POBTAS_PERMUTED 7
+#define ROCBTA_OP_POBTAF_STREAMING 8
+#define ROCBTA_OP_POBTASI_STREAMING 9
 
 /* Library version string. */
 const char* rocbta_version(void);
@@ -248,6 +250,31 @@ int rocbta_dpobtas_permuted(int64_t n, int64_t b, int64_t a, const double* D, co
                             const double* R, const double* T, const double* buffer, double* B,
                             int64_t r, int64_t ldb, char trans, void* workspace,
                             size_t workspace_bytes, void* stream);
+
+/* -------------------------------------------------------------- streaming
+ * Host <-> device streaming forms for systems whose blocks do not fit in
+ * device memory: D, L, R, T are HOST pointers (page-locked memory, e.g.
+ * hipHostMalloc, for asynchronous copies), overwritten in place with the
+ * same results as rocbta_dpobtaf / rocbta_dpobtasi.  Two device slots per
+ * block kind (workspace) carry the recurrence; the copies run on two streams
+ * of the call's own, overlapping the compute on `stream`.  Unlike the other
+ * entry points these calls are synchronous: they return when the host
+ * arrays hold the results (as the reference's streaming routines end with a
+ * device synchronisation).  `info` is a DEVICE int as for rocbta_dpobtaf.
+ * Replace serinv.algs.pobtaf(..., device_streaming=True) / _pobtaf_streaming
+ * (src/serinv/algs/pobtaf.py:284-524) and, with a == 0, _pobtf_streaming
+ * (pobtf.py:188-329); serinv.algs.pobtasi(..., device_streaming=True) /
+ * _pobtasi_streaming (pobtasi.py:247-496) and, with a == 0,
+ * _pobtsi_streaming (pobtsi.py:182-330).
+ * Workspace: rocbta_workspace_bytes(ROCBTA_OP_POBTAF_STREAMING /
+ * ROCBTA_OP_POBTASI_STREAMING, n, b, a, 0), about 6 b^2 / 12 b^2 doubles,
+ * independent of n. */
+int rocbta_dpobtaf_streaming(int64_t n, int64_t b, int64_t a, double* D, double* L, double* R,
+                             double* T, int factorize_last_block, int* info, void* workspace,
+                             size_t workspace_bytes, void* stream);
+int rocbta_dpobtasi_streaming(int64_t n, int64_t b, int64_t a, double* D, double* L, double* R,
+                              double* T, int invert_last_block, void* workspace,
+                              size_t workspace_bytes, void* stream);
 
 /* Diagnostic (no counterpart in the reference): runs the task decoding and
  * dependency enumeration of the persistent pobtaf kernel on the HOST for the

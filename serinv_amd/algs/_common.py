@@ -66,8 +66,23 @@ def check_kwargs(kwargs, expected):
 
 
 def reject_unsupported(kwargs, what):
+    """For the extensions without permuted or streaming forms (mixed)."""
+    if kwargs.get("buffer", None) is not None:
+        raise NotImplementedError(f"{what}: the permuted (buffer=...) form is not provided")
     if kwargs.get("device_streaming", False):
-        raise NotImplementedError(f"{what}: host-device streaming is not implemented")
+        raise NotImplementedError(f"{what}: host-device streaming is not provided")
+
+
+def streaming_requested(kwargs, what):
+    """device_streaming=True: the host-streaming form (natural ordering).
+    The permuted streaming forms (buffer= with device_streaming=True) are
+    not provided."""
+    if not kwargs.get("device_streaming", False):
+        return False
+    if kwargs.get("buffer", None) is not None:
+        raise NotImplementedError(f"{what}: host-device streaming of the permuted form is not "
+                                  "implemented")
+    return True
 
 
 def permutation_buffer(kwargs, D, min_blocks=1):

@@ -10,7 +10,8 @@ import torch
 
 from .. import _lib
 from ._common import (bta_shapes, check_kwargs, config, note_status, permutation_buffer, ptr,
-                      raise_on_info, reject_unsupported, stream_of, workspace)
+                      raise_on_info, stream_of, streaming_requested, workspace)
+from ._streaming import factor_streaming, host_shapes
 
 
 def pobtaf(A_diagonal_blocks, A_lower_diagonal_blocks, A_lower_arrow_blocks, A_arrow_tip_block,
@@ -27,14 +28,18 @@ def pobtaf(A_diagonal_blocks, A_lower_diagonal_blocks, A_lower_arrow_blocks, A_a
     serinv_amd.utils.allocate_pobtax_permutation_buffers) selects the
     permuted variant, which factors the middle blocks 1 .. n-2 and leaves the
     reduced system (blocks 0 and n-1, buffer[n-1], the tip) to the caller;
-    device_streaming is accepted but only its default is supported.
+    device_streaming=True takes HOST tensors (float64, CPU, best pinned) and
+    streams the blocks through the GPU (pobtaf.py:284-524).
     A non-positive pivot raises numpy.linalg.LinAlgError (when
     serinv_amd.algs.config["check_info"] is True).
     """
     check_kwargs(kwargs, {"device_streaming", "buffer", "factorize_last_block"})
-    reject_unsupported(kwargs, "pobtaf")
     factorize_last_block = bool(kwargs.get("factorize_last_block", True))
     D, L, R, T = A_diagonal_blocks, A_lower_diagonal_blocks, A_lower_arrow_blocks, A_arrow_tip_block
+    if streaming_requested(kwargs, "pobtaf"):
+        n, b, a = host_shapes(D, L, R, T)
+        factor_streaming(D, L, R, T, n, b, a, factorize_last_block, "pobtaf")
+        return
     n, b, a, dev = bta_shapes(D, L, R, T)
     buf = permutation_buffer(kwargs, D, min_blocks=2)
     if buf is not None:

@@ -6,8 +6,9 @@ buffer=); runs as rocbta_dpobtasi / rocbta_dpobtasi_permuted.
 """
 
 from .. import _lib
-from ._common import (bta_shapes, check_kwargs, permutation_buffer, ptr, reject_unsupported,
-                      stream_of, workspace)
+from ._common import (bta_shapes, check_kwargs, permutation_buffer, ptr, stream_of,
+                      streaming_requested, workspace)
+from ._streaming import host_shapes, selinv_streaming
 
 
 def pobtasi(L_diagonal_blocks, L_lower_diagonal_blocks, L_lower_arrow_blocks, L_arrow_tip_block,
@@ -18,13 +19,17 @@ def pobtasi(L_diagonal_blocks, L_lower_diagonal_blocks, L_lower_arrow_blocks, L_
     the blocks of A^{-1} on the BTA pattern.  Keyword arguments:
     invert_last_block (default True); buffer selects the permuted variant
     (the factor from pobtaf(..., buffer=) with the reduced system's blocks
-    already inverted; on exit buffer[i] = X_{0,i}); device_streaming only at
-    its default.
+    already inverted; on exit buffer[i] = X_{0,i}); device_streaming=True
+    takes HOST tensors and streams the blocks through the GPU
+    (pobtasi.py:247-496).
     """
     check_kwargs(kwargs, {"device_streaming", "buffer", "invert_last_block"})
-    reject_unsupported(kwargs, "pobtasi")
     invert_last_block = bool(kwargs.get("invert_last_block", True))
     D, L, R, T = L_diagonal_blocks, L_lower_diagonal_blocks, L_lower_arrow_blocks, L_arrow_tip_block
+    if streaming_requested(kwargs, "pobtasi"):
+        n, b, a = host_shapes(D, L, R, T)
+        selinv_streaming(D, L, R, T, n, b, a, invert_last_block, "pobtasi")
+        return
     n, b, a, dev = bta_shapes(D, L, R, T)
     buf = permutation_buffer(kwargs, D, min_blocks=2)
     if buf is not None:

@@ -5,7 +5,8 @@ Reference: src/serinv/algs/pobtf.py:11-80 (signature, kwargs, dispatch),
 rocbta_dpobtaf / rocbta_dpobtaf_permuted with a = 0.
 """
 
-from ._common import bta_shapes, check_kwargs, permutation_buffer, reject_unsupported
+from ._common import bta_shapes, check_kwargs, permutation_buffer, streaming_requested
+from ._streaming import factor_streaming, host_shapes
 from .pobtaf import _factor, _factor_permuted
 
 
@@ -16,9 +17,12 @@ def pobtf(A_diagonal_blocks, A_lower_diagonal_blocks, **kwargs):
     blocks (n-1, b, b) -> L_{i+1,i}.  Keyword arguments as serinv's pobtf.
     """
     check_kwargs(kwargs, {"device_streaming", "buffer", "factorize_last_block"})
-    reject_unsupported(kwargs, "pobtf")
     factorize_last_block = bool(kwargs.get("factorize_last_block", True))
     D, L = A_diagonal_blocks, A_lower_diagonal_blocks
+    if streaming_requested(kwargs, "pobtf"):  # pobtf.py:188-329
+        n, b, _ = host_shapes(D, L, None, None)
+        factor_streaming(D, L, None, None, n, b, 0, factorize_last_block, "pobtf")
+        return
     n, b, a, dev = bta_shapes(D, L, None, None)
     buf = permutation_buffer(kwargs, D, min_blocks=2)
     if buf is not None:  # pobtf.py:53-68, 125-185

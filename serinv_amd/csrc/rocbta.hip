@@ -960,6 +960,47 @@ PsiLayout psi_layout(int64_t n, int64_t b, int64_t a) {
   return l;
 }
 
+// Device workspace of the host-streaming forms: two slots of every block
+// kind in flight (one being computed, one being copied), the tip, and the
+// per-block scratch of the factorization / selected inversion.
+struct StLayout {
+  size_t d[2], l[2], r[2], xd[2], xl[2], xr[2], t, tinv, linv, w, g, gt, ti, wt, total;
+};
+StLayout st_layout(int64_t b, int64_t a, bool selinv) {
+  StLayout l{};
+  size_t off = 0;
+  auto take = [&](size_t bytes) {
+    size_t o = off;
+    off = align256(off + bytes);
+    return o;
+  };
+  const size_t bb = (size_t)b * b * sizeof(double), ab = (size_t)std::max<int64_t>(a, 1) * b * sizeof(double);
+  const size_t aa = (size_t)std::max<int64_t>(a, 1) * std::max<int64_t>(a, 1) * sizeof(double);
+  for (int k = 0; k < 2; ++k) {
+    l.d[k] = take(bb);
+    l.l[k] = take(bb);
+    l.r[k] = take(ab);
+  }
+  l.t = take(aa);
+  l.tinv = take(TT * TT * sizeof(double));
+  if (selinv) {
+    for (int k = 0; k < 2; ++k) {
+      l.xd[k] = take(bb);
+      l.xl[k] = take(bb);
+      l.xr[k] = take(ab);
+    }
+    const int64_t ntb = (b + TT - 1) / TT, nta = (std::max<int64_t>(a, 1) + TT - 1) / TT;
+    l.linv = take(bb);
+    l.w = take((size_t)std::max(ntb * TT * TT, ntb * TT * ntb * TT / 4) * sizeof(double));
+    l.g = take(bb);
+    l.gt = take(ab);
+    l.ti = take(aa);
+    l.wt = take((size_t)std::max(nta * TT * TT, nta * TT * nta * TT / 4) * sizeof(double));
+  }
+  l.total = off;
+  return l;
+}
+
 // dst = src^T for one b x b block (64 x 64 tiles through LDS).
 int transpose_block(const double* src, double* dst, int64_t b, hipStream_t s) {
   const int T = (int)((b + 63) / 64);
@@ -1007,6 +1048,10 @@ size_t rocbta_workspace_bytes(int op, int64_t n, int64_t b, int64_t a, int64_t r
       return psi_layout(n, b, a).total;
     case ROCBTA_OP_POBTAS_PERMUTED:
       return 256;
+    case ROCBTA_OP_POBTAF_STREAMING:
+      return st_layout(b, a, false).total;
+    case ROCBTA_OP_POBTASI_STREAMING:
+      return st_layout(b, a, true).total;
     case ROCBTA_OP_RESIDUAL: {
       const int64_t rr = std::max<int64_t>(r, 1), rc = sweep_rc(rr);
       return align256((size_t)cdiv(rr, rc) * n * cdiv(a, SW_T) * SW_T * rc * sizeof(double)) + 256;
@@ -1803,6 +1848,294 @@ int rocbta_dpobtas_permuted(int64_t n, int64_t b, int64_t a, const double* D, co
     if (rc) return rc;
   }
   return ROCBTA_OK;
+}
+
+// ------------------------------------------------------------- streaming
+// Host <-> device streaming forms: the block arrays stay in host memory
+// (pinned, for asynchronous copies), and two slots of device buffers per
+// block kind carry the recurrence, with the copies on two streams of their
+// own (host to device, device to host) overlapping the compute on the
+// caller's stream.  For problems whose blocks do not fit in device memory.
+// The calls return when the host arrays hold the results.
+namespace {
+struct CopyStreams {
+  hipStream_t h2d = nullptr, d2h = nullptr;
+  // ev[0..1]: h2d of slot k done; ev[2..3]: compute on slot k done;
+  // ev[4..5]: d2h of slot k done
+  hipEvent_t ev[6] = {};
+  int init(hipStream_t s) {
+    RB_CHECK(hipStreamCreateWithFlags(&h2d, hipStreamNonBlocking));
+    RB_CHECK(hipStreamCreateWithFlags(&d2h, hipStreamNonBlocking));
+    for (auto& e : ev) {
+      RB_CHECK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+      RB_CHECK(hipEventRecord(e, s));  // every wait below targets a recorded event
+    }
+    RB_CHECK(hipStreamWaitEvent(h2d, ev[0], 0));  // nothing starts before the caller's prior work
+    RB_CHECK(hipStreamWaitEvent(d2h, ev[0], 0));
+    return ROCBTA_OK;
+  }
+  // the caller's stream waits for both copy streams, then everything is done
+  int finish(hipStream_t s) {
+    hipEvent_t e1, e2;
+    RB_CHECK(hipEventCreateWithFlags(&e1, hipEventDisableTiming));
+    RB_CHECK(hipEventCreateWithFlags(&e2, hipEventDisableTiming));
+    RB_CHECK(hipEventRecord(e1, h2d));
+    RB_CHECK(hipEventRecord(e2, d2h));
+    RB_CHECK(hipStreamWaitEvent(s, e1, 0));
+    RB_CHECK(hipStreamWaitEvent(s, e2, 0));
+    RB_CHECK(hipStreamSynchronize(s));
+    (void)hipEventDestroy(e1);
+    (void)hipEventDestroy(e2);
+    return ROCBTA_OK;
+  }
+  // release (after the copy streams drained, also on an error return)
+  ~CopyStreams() {
+    if (h2d) (void)hipStreamSynchronize(h2d);
+    if (d2h) (void)hipStreamSynchronize(d2h);
+    for (auto& e : ev)
+      if (e) (void)hipEventDestroy(e);
+    if (h2d) (void)hipStreamDestroy(h2d);
+    if (d2h) (void)hipStreamDestroy(d2h);
+  }
+};
+
+int h2d_copy(double* dst, const double* src, int64_t count, hipStream_t s) {
+  if (count <= 0) return ROCBTA_OK;
+  RB_CHECK(hipMemcpyAsync(dst, src, count * sizeof(double), hipMemcpyHostToDevice, s));
+  return ROCBTA_OK;
+}
+int d2h_copy(double* dst, const double* src, int64_t count, hipStream_t s) {
+  if (count <= 0) return ROCBTA_OK;
+  RB_CHECK(hipMemcpyAsync(dst, src, count * sizeof(double), hipMemcpyDeviceToHost, s));
+  return ROCBTA_OK;
+}
+}  // namespace
+
+int rocbta_dpobtaf_streaming(int64_t n, int64_t b, int64_t a, double* D, double* L, double* R,
+                             double* T, int factorize_last_block, int* info, void* workspace,
+                             size_t workspace_bytes, void* stream) {
+  if (bad_shape(n, b, a) || !D || !info || (n > 1 && !L) || (a > 0 && (!R || !T)))
+    return ROCBTA_ERR_ARG;
+  const StLayout lay = st_layout(b, a, false);
+  if (!workspace || workspace_bytes < lay.total) return ROCBTA_ERR_WORKSPACE;
+  hipStream_t s = (hipStream_t)stream;
+  char* ws = (char*)workspace;
+  double *Dd[2], *Ld[2], *Rd[2];
+  for (int k = 0; k < 2; ++k) {
+    Dd[k] = (double*)(ws + lay.d[k]);
+    Ld[k] = (double*)(ws + lay.l[k]);
+    Rd[k] = (double*)(ws + lay.r[k]);
+  }
+  double* Td = (double*)(ws + lay.t);
+  double* tinv = (double*)(ws + lay.tinv);
+  const int bi = (int)b, ai = (int)a;
+  const int64_t bb = b * b, ab = a * b;
+  set_int_kernel<<<1, 64, 0, s>>>(info, 0);
+  RB_CHECK(hipGetLastError());
+  CopyStreams cs;
+  int rc = cs.init(s);
+  if (rc) return rc;
+  // pobtaf.py:343-362: the tip and block 0 in
+  rc = h2d_copy(Td, T, a * a, cs.h2d);
+  if (!rc) rc = h2d_copy(Dd[0], D, bb, cs.h2d);
+  if (!rc) rc = h2d_copy(Rd[0], R, ab, cs.h2d);
+  if (!rc && n > 1) rc = h2d_copy(Ld[0], L, bb, cs.h2d);
+  if (rc) return rc;
+  RB_CHECK(hipEventRecord(cs.ev[0], cs.h2d));
+  RB_CHECK(hipStreamWaitEvent(s, cs.ev[0], 0));
+  for (int64_t i = 0; i + 1 < n; ++i) {  // pobtaf.py:364-467
+    const int k = (int)(i & 1), k1 = k ^ 1;
+    // block i+1 into the other slot, once its previous contents are out
+    RB_CHECK(hipStreamWaitEvent(cs.h2d, cs.ev[4 + k1], 0));
+    rc = h2d_copy(Dd[k1], D + (i + 1) * bb, bb, cs.h2d);
+    if (!rc) rc = h2d_copy(Rd[k1], R + (i + 1) * ab, ab, cs.h2d);
+    if (!rc && i + 2 < n) rc = h2d_copy(Ld[k1], L + (i + 1) * bb, bb, cs.h2d);
+    if (rc) return rc;
+    RB_CHECK(hipEventRecord(cs.ev[k1], cs.h2d));
+    // L_ii, L_{i+1,i}, L_{t,i}
+    std::vector<Panel> panels = {{Ld[k], bi, b}};
+    if (a > 0) panels.push_back({Rd[k], ai, b});
+    rc = factor_block_column(Dd[k], b, bi, panels, i * b, info, tinv, s);
+    if (rc) return rc;
+    // Schur updates of block i+1 and the tip (pobtaf.py:443-467)
+    RB_CHECK(hipStreamWaitEvent(s, cs.ev[k1], 0));
+    const bool full_last = (i + 2 == n) && !factorize_last_block;
+    GemmList upd;
+    push_term(upd.add(Dd[k1], b, bi, bi, 1.0, full_last ? TRI_NONE : TRI_LOWER),
+              term(Ld[k], b, false, Ld[k], b, true, bi, -1.0));
+    if (a > 0) {
+      push_term(upd.add(Rd[k1], b, ai, bi, 1.0), term(Rd[k], b, false, Ld[k], b, true, bi, -1.0));
+      push_term(upd.add(Td, a, ai, ai, 1.0), term(Rd[k], b, false, Rd[k], b, true, bi, -1.0));
+    }
+    rc = run_gemms(upd, s);
+    if (rc) return rc;
+    RB_CHECK(hipEventRecord(cs.ev[2 + k], s));
+    // factor blocks of step i out
+    RB_CHECK(hipStreamWaitEvent(cs.d2h, cs.ev[2 + k], 0));
+    rc = d2h_copy(D + i * bb, Dd[k], bb, cs.d2h);
+    if (!rc) rc = d2h_copy(L + i * bb, Ld[k], bb, cs.d2h);
+    if (!rc) rc = d2h_copy(R + i * ab, Rd[k], ab, cs.d2h);
+    if (rc) return rc;
+    RB_CHECK(hipEventRecord(cs.ev[4 + k], cs.d2h));
+  }
+  // last block and tip (pobtaf.py:469-522)
+  const int kl = (int)((n - 1) & 1);
+  if (factorize_last_block) {
+    std::vector<Panel> panels;
+    if (a > 0) panels.push_back({Rd[kl], ai, b});
+    rc = factor_block_column(Dd[kl], b, bi, panels, (n - 1) * b, info, tinv, s);
+    if (rc) return rc;
+    if (a > 0) {
+      GemmList upd;
+      push_term(upd.add(Td, a, ai, ai, 1.0), term(Rd[kl], b, false, Rd[kl], b, true, bi, -1.0));
+      rc = run_gemms(upd, s);
+      if (!rc) rc = factor_block_column(Td, a, ai, {}, n * b, info, tinv, s);
+      if (rc) return rc;
+    }
+  }
+  RB_CHECK(hipEventRecord(cs.ev[2 + kl], s));
+  RB_CHECK(hipStreamWaitEvent(cs.d2h, cs.ev[2 + kl], 0));
+  rc = d2h_copy(D + (n - 1) * bb, Dd[kl], bb, cs.d2h);
+  if (!rc) rc = d2h_copy(R + (n - 1) * ab, Rd[kl], ab, cs.d2h);
+  if (!rc) rc = d2h_copy(T, Td, a * a, cs.d2h);
+  if (rc) return rc;
+  return cs.finish(s);
+}
+
+int rocbta_dpobtasi_streaming(int64_t n, int64_t b, int64_t a, double* D, double* L, double* R,
+                              double* T, int invert_last_block, void* workspace,
+                              size_t workspace_bytes, void* stream) {
+  if (bad_shape(n, b, a) || !D || (n > 1 && !L) || (a > 0 && (!R || !T))) return ROCBTA_ERR_ARG;
+  const StLayout lay = st_layout(b, a, true);
+  if (!workspace || workspace_bytes < lay.total) return ROCBTA_ERR_WORKSPACE;
+  hipStream_t s = (hipStream_t)stream;
+  char* ws = (char*)workspace;
+  double *Dd[2], *Ld[2], *Rd[2], *XD[2], *XL[2], *XR[2];
+  for (int k = 0; k < 2; ++k) {
+    Dd[k] = (double*)(ws + lay.d[k]);
+    Ld[k] = (double*)(ws + lay.l[k]);
+    Rd[k] = (double*)(ws + lay.r[k]);
+    XD[k] = (double*)(ws + lay.xd[k]);
+    XL[k] = (double*)(ws + lay.xl[k]);
+    XR[k] = (double*)(ws + lay.xr[k]);
+  }
+  double* Td = (double*)(ws + lay.t);
+  double* Linv = (double*)(ws + lay.linv);
+  double* W = (double*)(ws + lay.w);
+  double* G = (double*)(ws + lay.g);
+  double* Gt = (double*)(ws + lay.gt);
+  double* Ti = (double*)(ws + lay.ti);
+  double* Wt = (double*)(ws + lay.wt);
+  const int bi = (int)b, ai = (int)a;
+  const int64_t bb = b * b, ab = a * b;
+  CopyStreams cs;
+  int rc = cs.init(s);
+  if (rc) return rc;
+  const int kl = (int)((n - 1) & 1);
+  // tip and last block in (pobtasi.py:318-333)
+  rc = h2d_copy(Td, T, a * a, cs.h2d);
+  if (!rc) rc = h2d_copy(invert_last_block ? Dd[kl] : XD[kl], D + (n - 1) * bb, bb, cs.h2d);
+  if (!rc) rc = h2d_copy(invert_last_block ? Rd[kl] : XR[kl], R + (n - 1) * ab, ab, cs.h2d);
+  if (rc) return rc;
+  RB_CHECK(hipEventRecord(cs.ev[kl], cs.h2d));
+  RB_CHECK(hipStreamWaitEvent(s, cs.ev[kl], 0));
+  if (invert_last_block) {  // pobtasi.py:335-404
+    if (a > 0) {
+      rc = block_trtri(Td, a, 0, Ti, a, a * a, ai, 1, Wt, s);
+      if (rc) return rc;
+      GemmList xt;
+      push_term(xt.add(Td, a, ai, ai, 0.0), term(Ti, a, true, Ti, a, false, ai, 1.0, TRI_UPPER,
+                                                  TRI_LOWER));
+      rc = run_gemms(xt, s);
+      if (rc) return rc;
+    }
+    rc = block_trtri(Dd[kl], b, 0, Linv, b, bb, bi, 1, W, s, true);
+    if (rc) return rc;
+    GemmList g1;
+    GemmProblem& ps = g1.add(XD[kl], b, bi, bi, 0.0, TRI_LOWER);
+    ps.mirror = 1;
+    push_term(ps, term(Linv, b, true, Linv, b, false, bi, 1.0, TRI_UPPER, TRI_LOWER));
+    if (a > 0)
+      push_term(g1.add(Gt, b, ai, bi, 0.0),
+                term(Rd[kl], b, false, Linv, b, false, bi, 1.0, TRI_NONE, TRI_LOWER));
+    rc = run_gemms(g1, s);
+    if (rc) return rc;
+    if (a > 0) {
+      GemmList g2;
+      push_term(g2.add(XR[kl], b, ai, bi, 0.0), term(Td, a, false, Gt, b, false, ai, -1.0));
+      rc = run_gemms(g2, s);
+      if (rc) return rc;
+      GemmList g3;
+      GemmProblem& pd = g3.add(XD[kl], b, bi, bi, 1.0, TRI_LOWER);
+      pd.mirror = 1;
+      push_term(pd, term(XR[kl], b, true, Gt, b, false, ai, -1.0));
+      rc = run_gemms(g3, s);
+      if (rc) return rc;
+    }
+  }
+  RB_CHECK(hipEventRecord(cs.ev[2 + kl], s));
+  RB_CHECK(hipStreamWaitEvent(cs.d2h, cs.ev[2 + kl], 0));
+  rc = d2h_copy(D + (n - 1) * bb, XD[kl], bb, cs.d2h);
+  if (!rc) rc = d2h_copy(R + (n - 1) * ab, XR[kl], ab, cs.d2h);
+  if (rc) return rc;
+  RB_CHECK(hipEventRecord(cs.ev[4 + kl], cs.d2h));
+  for (int64_t i = n - 2; i >= 0; --i) {  // pobtasi.py:406-494
+    const int k = (int)(i & 1), k1 = k ^ 1;
+    // factor blocks of step i in, once slot k's last compute (step i+2) is done
+    RB_CHECK(hipStreamWaitEvent(cs.h2d, cs.ev[2 + k], 0));
+    rc = h2d_copy(Dd[k], D + i * bb, bb, cs.h2d);
+    if (!rc) rc = h2d_copy(Ld[k], L + i * bb, bb, cs.h2d);
+    if (!rc) rc = h2d_copy(Rd[k], R + i * ab, ab, cs.h2d);
+    if (rc) return rc;
+    RB_CHECK(hipEventRecord(cs.ev[k], cs.h2d));
+    RB_CHECK(hipStreamWaitEvent(s, cs.ev[k], 0));
+    // the X blocks of slot k are rewritten: their copy-out (step i+2) first
+    RB_CHECK(hipStreamWaitEvent(s, cs.ev[4 + k], 0));
+    rc = block_trtri(Dd[k], b, 0, Linv, b, bb, bi, 1, W, s, true);
+    if (rc) return rc;
+    // G = L_{i+1,i} L_ii^{-1}, Gt = L_{t,i} L_ii^{-1}, X_ii <- L_ii^{-T} L_ii^{-1}
+    GemmList g1;
+    push_term(g1.add(G, b, bi, bi, 0.0),
+              term(Ld[k], b, false, Linv, b, false, bi, 1.0, TRI_NONE, TRI_LOWER));
+    if (a > 0)
+      push_term(g1.add(Gt, b, ai, bi, 0.0),
+                term(Rd[k], b, false, Linv, b, false, bi, 1.0, TRI_NONE, TRI_LOWER));
+    GemmProblem& ps = g1.add(XD[k], b, bi, bi, 0.0, TRI_LOWER);
+    ps.mirror = 1;
+    push_term(ps, term(Linv, b, true, Linv, b, false, bi, 1.0, TRI_UPPER, TRI_LOWER));
+    rc = run_gemms(g1, s);
+    if (rc) return rc;
+    // X_{i+1,i} = -X_{i+1,i+1} G - X_{t,i+1}^T Gt ; X_{t,i} = -X_{t,i+1} G - X_T Gt
+    GemmList g2;
+    GemmProblem& pl = g2.add(XL[k], b, bi, bi, 0.0);
+    push_term(pl, term(XD[k1], b, false, G, b, false, bi, -1.0));
+    if (a > 0) {
+      push_term(pl, term(XR[k1], b, true, Gt, b, false, ai, -1.0));
+      GemmProblem& pr = g2.add(XR[k], b, ai, bi, 0.0);
+      push_term(pr, term(XR[k1], b, false, G, b, false, bi, -1.0));
+      push_term(pr, term(Td, a, false, Gt, b, false, ai, -1.0));
+    }
+    rc = run_gemms(g2, s);
+    if (rc) return rc;
+    // X_ii += -X_{i+1,i}^T G - X_{t,i}^T Gt (symmetric: lower tiles, mirrored)
+    GemmList g3;
+    GemmProblem& pd = g3.add(XD[k], b, bi, bi, 1.0, TRI_LOWER);
+    pd.mirror = 1;
+    push_term(pd, term(XL[k], b, true, G, b, false, bi, -1.0));
+    if (a > 0) push_term(pd, term(XR[k], b, true, Gt, b, false, ai, -1.0));
+    rc = run_gemms(g3, s);
+    if (rc) return rc;
+    RB_CHECK(hipEventRecord(cs.ev[2 + k], s));
+    RB_CHECK(hipStreamWaitEvent(cs.d2h, cs.ev[2 + k], 0));
+    rc = d2h_copy(D + i * bb, XD[k], bb, cs.d2h);
+    if (!rc) rc = d2h_copy(L + i * bb, XL[k], bb, cs.d2h);
+    if (!rc) rc = d2h_copy(R + i * ab, XR[k], ab, cs.d2h);
+    if (rc) return rc;
+    RB_CHECK(hipEventRecord(cs.ev[4 + k], cs.d2h));
+  }
+  rc = d2h_copy(T, Td, a * a, cs.d2h);
+  if (rc) return rc;
+  return cs.finish(s);
 }
 
 }  // extern "C"

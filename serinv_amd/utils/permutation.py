@@ -18,7 +18,7 @@ def allocate_pobtx_permutation_buffers(A_diagonal_blocks, device_streaming=False
     NotImplementedError.
     """
     if device_streaming:
-        raise NotImplementedError("host-device streaming is not implemented")
+        raise NotImplementedError("host-device streaming of the permuted forms is not provided")
     if not isinstance(A_diagonal_blocks, torch.Tensor):
         raise TypeError("A_diagonal_blocks must be a torch.Tensor on a ROCm device")
     return torch.empty_like(A_diagonal_blocks, memory_format=torch.contiguous_format)

@@ -109,3 +109,22 @@ def test_permuted_workspace_and_argument_errors(lib):
                                        None, 0, None) == _lib.ROCBTA_ERR_ARG
     assert lib.rocbta_dpobtas_permuted(3, 4, 1, fake, fake, fake, fake, fake, fake, 2, 1, b"N",
                                        None, 0, None) == _lib.ROCBTA_ERR_ARG  # ldb < r
+
+
+def test_streaming_workspace_and_argument_errors(lib):
+    b, a = 1024, 32
+    # two slots per block kind, independent of n
+    f = lib.rocbta_workspace_bytes(_lib.OP_POBTAF_STREAMING, 256, b, a, 0)
+    assert f >= 4 * b * b * 8 and f == lib.rocbta_workspace_bytes(_lib.OP_POBTAF_STREAMING, 4, b, a, 0)
+    assert lib.rocbta_workspace_bytes(_lib.OP_POBTASI_STREAMING, 256, b, a, 0) >= 10 * b * b * 8
+    fake = ctypes.c_void_p(16)
+    assert lib.rocbta_dpobtaf_streaming(3, 4, 1, None, fake, fake, fake, 1, fake, fake, 1 << 20,
+                                        None) == _lib.ROCBTA_ERR_ARG
+    assert lib.rocbta_dpobtaf_streaming(3, 4, 1, fake, fake, fake, fake, 1, None, fake, 1 << 20,
+                                        None) == _lib.ROCBTA_ERR_ARG
+    assert lib.rocbta_dpobtaf_streaming(3, 4, 1, fake, fake, fake, fake, 1, fake, fake, 8,
+                                        None) == _lib.ROCBTA_ERR_WORKSPACE
+    assert lib.rocbta_dpobtasi_streaming(3, 4, 1, fake, fake, None, fake, 1, fake, 1 << 20,
+                                         None) == _lib.ROCBTA_ERR_ARG
+    assert lib.rocbta_dpobtasi_streaming(3, 4, 1, fake, fake, fake, fake, 1, fake, 8,
+                                         None) == _lib.ROCBTA_ERR_WORKSPACE
