@@ -44,6 +44,7 @@ def parse():
     p.add_argument("--no-batched", action="store_true")
     p.add_argument("--no-mixed", action="store_true")
     p.add_argument("--no-permuted", action="store_true")
+    p.add_argument("--no-streaming", action="store_true")
     return p.parse_args()
 
 
@@ -326,6 +327,53 @@ def bench_permuted(dev, n, b, a, reps=2):
     return res
 
 
+def bench_streaming(dev, n, b, a, reps=2):
+    """Host-streaming pobtaf + pobtasi (device_streaming=True, page-locked
+    host blocks) against the in-memory calls on the same matrix; both give
+    the same factor and selected inverse (largest relative difference
+    reported).  Times include every host <-> device copy."""
+    import time
+
+    import torch
+
+    import serinv_amd.algs as algs
+    from serinv_amd.utils.matrices import bta_factor_bytes, bta_spd
+
+    host = [torch.from_numpy(x).pin_memory() for x in bta_spd(n, b, a, seed=63)[:4]]
+    res = {"n": n, "b": b, "a": a}
+    t_af, t_si = [], []
+    for _ in range(reps):
+        D, L, R, T = (x.clone().pin_memory() for x in host)
+        t0 = time.perf_counter()
+        algs.pobtaf(D, L, R, T, device_streaming=True)
+        t1 = time.perf_counter()
+        algs.pobtasi(D, L, R, T, device_streaming=True)
+        t2 = time.perf_counter()
+        t_af.append(t1 - t0)
+        t_si.append(t2 - t1)
+    Dg, Lg, Rg, Tg = (x.to(dev) for x in host)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    algs.pobtaf(Dg, Lg, Rg, Tg)
+    algs.pobtasi(Dg, Lg, Rg, Tg)
+    torch.cuda.synchronize()
+    t_mem = time.perf_counter() - t0
+
+    def rel(x, y):
+        y = y.cpu()
+        return float((x - y).abs().max() / y.abs().max()) if y.numel() else 0.0
+
+    res["streaming_pobtaf_ms"] = round(min(t_af) * 1e3, 3)
+    res["streaming_pobtasi_ms"] = round(min(t_si) * 1e3, 3)
+    res["in_memory_pobtaf_pobtasi_ms"] = round(t_mem * 1e3, 3)
+    # each routine moves every block in and out once
+    res["host_link_GBs_per_direction"] = round(
+        bta_factor_bytes(n, b, a) / (min(t_af) + min(t_si)) * 2 * 1e-9, 1)
+    res["max_rel_diff_vs_in_memory"] = float(
+        f"{max(rel(D, Dg), rel(L, Lg), rel(R, Rg), rel(T, Tg)):.3e}")
+    return res
+
+
 def bench_mixed(dev, n, b, a, kappas=(1e4, 1e6, 1e8), r=16, reps=2):
     """Mixed FP32/FP64 against plain FP64 on the calibrated matrices of the
     given shape (cond ~ kappa): pobtaf + pobtasi time of both forms, max
@@ -575,6 +623,10 @@ def main():
         print("[bench] permuted (middle-partition) form, n=64 ...", file=sys.stderr, flush=True)
         out["permuted"] = bench_permuted(dev, 64, b, a)
         print(f"[bench] permuted {out['permuted']}", file=sys.stderr, flush=True)
+    if not args.no_streaming:
+        print("[bench] host-streaming form, n=64 ...", file=sys.stderr, flush=True)
+        out["streaming"] = bench_streaming(dev, 64, b, a)
+        print(f"[bench] streaming {out['streaming']}", file=sys.stderr, flush=True)
     if not args.no_batched:
         print("[bench] batched plan vs per-matrix loop ...", file=sys.stderr, flush=True)
         out["batched"] = bench_batched(dev)
