@@ -45,6 +45,8 @@ extern "C" {
 #define ROCBTA_OP_POBTAS_PERMUTED 7
 #define ROCBTA_OP_POBTAF_STREAMING 8
 #define ROCBTA_OP_POBTASI_STREAMING 9
+#define ROCBTA_OP_DDBTASC 10
+#define ROCBTA_OP_DDBTASCI 11
 
 /* Library version string. */
 const char* rocbta_version(void);
@@ -250,6 +252,32 @@ int rocbta_dpobtas_permuted(int64_t n, int64_t b, int64_t a, const double* D, co
                             const double* R, const double* T, const double* buffer, double* B,
                             int64_t r, int64_t ldb, char trans, void* workspace,
                             size_t workspace_bytes, void* stream);
+
+/* --------------------------------------------------------- non-SPD ddbt*
+ * Diagonally dominant BTA / BT matrices without symmetry.  Arrays (device):
+ *   D (n, b, b), Lo (n-1, b, b) = A_{i+1,i}, Up (n-1, b, b) = A_{i,i+1},
+ *   Rl (n, a, b) = A_{t,i}, Ru (n, b, a) = A_{i,t}, T (a, a).
+ * Block inverses are blocked Gauss-Jordan without pivoting (stable for
+ * diagonally dominant blocks; the reference calls numpy/cupy linalg.inv).
+ * direction: 0 = downward; 1 = upward (a == 0 only, as the reference's BT
+ * routines).  Only the "AX=I" forms (no rhs / quadratic / buffers).
+ * Workspace: rocbta_workspace_bytes(ROCBTA_OP_DDBTASC / _DDBTASCI, n, b, a, 0).
+ *
+ * Schur-complement sweep, in place: D[i] <- inverse of the Schur complement
+ * of block i, Rl / Ru / T Schur-updated, T <- the tip's inverse (with
+ * invert_last_block).  Replaces serinv.algs.ddbtasc / _ddbtasc
+ * (src/serinv/algs/ddbtasc.py:9-185, 187-241) and, with a == 0,
+ * ddbtsc / _ddbtsc / _ddbtsc_upward (ddbtsc.py:9-221). */
+int rocbta_dddbtasc(int64_t n, int64_t b, int64_t a, double* D, double* Lo, double* Up,
+                    double* Rl, double* Ru, double* T, int invert_last_block, int direction,
+                    void* workspace, size_t workspace_bytes, void* stream);
+/* Selected inversion from the sweep's output: the blocks of A^{-1} on the
+ * pattern.  Replaces serinv.algs.ddbtasci / _ddbtasci (ddbtasci.py:9-185,
+ * 187-256) and, with a == 0, ddbtsci / _ddbtsci / _ddbtsci_upward
+ * (ddbtsci.py:9-237). */
+int rocbta_dddbtasci(int64_t n, int64_t b, int64_t a, double* D, double* Lo, double* Up,
+                     double* Rl, double* Ru, double* T, int invert_last_block, int direction,
+                     void* workspace, size_t workspace_bytes, void* stream);
 
 /* -------------------------------------------------------------- streaming
  * Host <-> device streaming forms for systems whose blocks do not fit in

@@ -61,3 +61,29 @@ def bta_dense_to_arrays(A, b, a, n):
 def bt_dense_to_arrays(A, b, n):
     D, L, _, _ = bta_dense_to_arrays(A, b, 0, n)
     return D, L
+
+
+def bta_dense_to_arrays_full(A, b, a, n):
+    """(D, Lo, Up, Rl, Ru, T) of a general (non-symmetric) BTA matrix:
+    Lo[i] = A_{i+1,i}, Up[i] = A_{i,i+1}, Rl[i] = A_{t,i}, Ru[i] = A_{i,t}
+    (the arrays serinv's tests/testing_utils.py:17-122 returns for the ddbt
+    routines)."""
+    D = np.zeros((n, b, b), dtype=A.dtype)
+    Lo = np.zeros((max(n - 1, 0), b, b), dtype=A.dtype)
+    Up = np.zeros((max(n - 1, 0), b, b), dtype=A.dtype)
+    Rl = np.zeros((n, a, b), dtype=A.dtype)
+    Ru = np.zeros((n, b, a), dtype=A.dtype)
+    T = np.zeros((a, a), dtype=A.dtype)
+    for i in range(n):
+        blk = slice(i * b, (i + 1) * b)
+        D[i] = A[blk, blk]
+        if i + 1 < n:
+            nxt = slice((i + 1) * b, (i + 2) * b)
+            Lo[i] = A[nxt, blk]
+            Up[i] = A[blk, nxt]
+        if a:
+            Rl[i] = A[n * b:, blk]
+            Ru[i] = A[blk, n * b:]
+    if a:
+        T[:] = A[n * b:, n * b:]
+    return D, Lo, Up, Rl, Ru, T

@@ -299,3 +299,85 @@ def logdet_from_factor(D, T=None):
     if T is not None and T.size:
         s += 2.0 * np.sum(np.log(np.diag(T)))
     return float(s)
+
+
+# --------------------------------------------------------------------------
+# Non-SPD (diagonally dominant) family: ddbtasc / ddbtasci, ddbtsc / ddbtsci.
+# Block arrays: diagonal (n,b,b), lower (n-1,b,b) = A_{i+1,i}, upper
+# (n-1,b,b) = A_{i,i+1}, lower arrow (n,a,b) = A_{t,i}, upper arrow (n,b,a)
+# = A_{i,t}, tip (a,a).  The block inverses are numpy.linalg.inv, which the
+# reference calls (LU with partial pivoting).  Only the "AX=I" forms
+# (no rhs, no buffers) are restated.
+# --------------------------------------------------------------------------
+def ddbtasc(D, Lo, Up, Rl, Ru, T, invert_last_block=True):
+    """Schur-complement (block LU) sweep, src/serinv/algs/ddbtasc.py:187-241."""
+    n = D.shape[0]
+    for i in range(1, n):  # :199
+        D[i - 1] = np.linalg.inv(D[i - 1])  # :201
+        D[i] = D[i] - Lo[i - 1] @ D[i - 1] @ Up[i - 1]  # :204-209
+        Rl[i] = Rl[i] - Rl[i - 1] @ D[i - 1] @ Up[i - 1]  # :212-217
+        Ru[i] = Ru[i] - Lo[i - 1] @ D[i - 1] @ Ru[i - 1]  # :220-225
+        T[:, :] = T - Rl[i - 1] @ D[i - 1] @ Ru[i - 1]  # :228-233
+    if invert_last_block:  # :235-241
+        D[-1] = np.linalg.inv(D[-1])
+        T[:, :] = np.linalg.inv(T - Rl[-1] @ D[-1] @ Ru[-1])
+
+
+def ddbtasci(D, Lo, Up, Rl, Ru, T, invert_last_block=True):
+    """Selected inversion from the ddbtasc output, src/serinv/algs/ddbtasci.py:187-256."""
+    n = D.shape[0]
+    if invert_last_block:  # :207-219
+        B2 = D[-1] @ Ru[-1]
+        Rl[-1] = -T @ Rl[-1] @ D[-1]
+        Ru[-1] = -B2 @ T
+        D[-1] = D[-1] - B2 @ Rl[-1]
+    for i in range(n - 2, -1, -1):  # :221
+        B1 = Up[i] @ D[i + 1] + Ru[i] @ Rl[i + 1]  # :222-225
+        B2 = Up[i] @ Ru[i + 1] + Ru[i] @ T  # :227-230
+        C1 = D[i + 1] @ Lo[i] + Ru[i + 1] @ Rl[i]  # :232-235
+        C2 = Rl[i + 1] @ Lo[i] + T @ Rl[i]  # :237-240
+        Up[i] = -D[i] @ B1  # :242
+        Ru[i] = -D[i] @ B2  # :243
+        D1 = Lo[i].copy()  # :245
+        D2 = Rl[i].copy()  # :246
+        Lo[i] = -C1 @ D[i]  # :248
+        Rl[i] = -C2 @ D[i]  # :249
+        D[i] = D[i] + D[i] @ (B1 @ D1 + B2 @ D2) @ D[i]  # :251-256
+
+
+def ddbtsc(D, Lo, Up, invert_last_block=True, direction="downward"):
+    """src/serinv/algs/ddbtsc.py:180-221 (downward and upward)."""
+    n = D.shape[0]
+    if direction == "downward":
+        for i in range(0, n - 1):  # :188
+            D[i] = np.linalg.inv(D[i])
+            D[i + 1] = D[i + 1] - Lo[i] @ D[i] @ Up[i]
+        if invert_last_block:
+            D[-1] = np.linalg.inv(D[-1])
+    elif direction == "upward":
+        for i in range(n - 1, 0, -1):  # :210
+            D[i] = np.linalg.inv(D[i])
+            D[i - 1] = D[i - 1] - Up[i - 1] @ D[i] @ Lo[i - 1]
+        if invert_last_block:
+            D[0] = np.linalg.inv(D[0])
+    else:
+        raise ValueError("Optional keyword argument `direction` must be either 'downward' or 'upward'")
+
+
+def ddbtsci(D, Lo, Up, direction="downward"):
+    """src/serinv/algs/ddbtsci.py:176-237 (downward and upward)."""
+    n = D.shape[0]
+    if direction == "downward":
+        for i in range(n - 2, -1, -1):  # :188
+            tl = Lo[i].copy()
+            Lo[i] = -D[i + 1] @ Lo[i] @ D[i]
+            Up[i] = -D[i] @ Up[i] @ D[i + 1]
+            D[i] = D[i] - Up[i] @ tl @ D[i]
+    elif direction == "upward":
+        for i in range(1, n):  # :220
+            tu = Up[i - 1].copy()
+            Lo[i - 1] = -D[i] @ Lo[i - 1] @ D[i - 1]
+            Up[i - 1] = -D[i - 1] @ Up[i - 1] @ D[i]
+            D[i] = D[i] - Lo[i - 1] @ tu @ D[i]
+    else:
+        raise ValueError("Optional keyword argument `direction` must be either 'downward' or 'upward'")

@@ -18,6 +18,7 @@ ROCBTA_ERR_WORKSPACE = -3
 OP_POBTAF, OP_POBTASI, OP_POBTAS, OP_RESIDUAL, OP_LOGDET = 0, 1, 2, 3, 4
 OP_POBTAF_PERMUTED, OP_POBTASI_PERMUTED, OP_POBTAS_PERMUTED = 5, 6, 7
 OP_POBTAF_STREAMING, OP_POBTASI_STREAMING = 8, 9
+OP_DDBTASC, OP_DDBTASCI = 10, 11
 BATCHED_FACTOR, BATCHED_SOLVE, BATCHED_SELINV, BATCHED_MIXED = 1, 2, 4, 8
 BATCHED_RC = {1: 16, 4: 32, 8: 64}  # forced sweep column-chunk width
 BATCHED_GEMM512 = 128
@@ -56,6 +57,10 @@ SIGNATURES = {
                                           _c_p, _c_size, _c_p]),
     "rocbta_dpobtasi_streaming": (_c_int, [_c_i64, _c_i64, _c_i64, _c_p, _c_p, _c_p, _c_p, _c_int,
                                            _c_p, _c_size, _c_p]),
+    "rocbta_dddbtasc": (_c_int, [_c_i64, _c_i64, _c_i64, _c_p, _c_p, _c_p, _c_p, _c_p, _c_p, _c_int,
+                                 _c_int, _c_p, _c_size, _c_p]),
+    "rocbta_dddbtasci": (_c_int, [_c_i64, _c_i64, _c_i64, _c_p, _c_p, _c_p, _c_p, _c_p, _c_p, _c_int,
+                                  _c_int, _c_p, _c_size, _c_p]),
     "rocbta_dlogdet": (_c_int, [_c_i64, _c_i64, _c_i64, _c_p, _c_p, _c_p, _c_p, _c_size, _c_p]),
     "rocbta_dag_selfcheck": (_c_i64, [_c_i64, _c_i64, _c_i64]),
     "rocbta_mpobtaf": (_c_int, [_c_i64, _c_i64, _c_i64, _c_p, _c_p, _c_p, _c_p, _c_int, _c_p,

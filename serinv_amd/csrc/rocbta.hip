@@ -21,6 +21,7 @@
 #include "solve_block.hpp"
 #include "resid_f64.hpp"
 #include "tile_f64.hpp"
+#include "getri_f64.hpp"
 
 #include <atomic>
 #include <cstdlib>
@@ -1001,6 +1002,87 @@ StLayout st_layout(int64_t b, int64_t a, bool selinv) {
   return l;
 }
 
+// Workspace of the non-SPD sweeps: the blocked inverse's pivot-tile
+// inverse P and its row / column panels, and the products of one step.
+struct DdLayout {
+  size_t p, r, c, w, v, b1, b2, c1, c2, e, g, dc, y, total;
+};
+DdLayout dd_layout(int64_t b, int64_t a) {
+  DdLayout l{};
+  size_t off = 0;
+  auto take = [&](size_t bytes) {
+    size_t o = off;
+    off = align256(off + bytes);
+    return o;
+  };
+  const int64_t mm = std::max(b, a), a1 = std::max<int64_t>(a, 1);
+  const size_t bb = (size_t)b * b * sizeof(double), ab = (size_t)a1 * b * sizeof(double);
+  l.p = take(TT * TT * sizeof(double));
+  l.r = take((size_t)TT * mm * sizeof(double));
+  l.c = take((size_t)mm * TT * sizeof(double));
+  l.w = take(bb);
+  l.v = take(ab);
+  l.b1 = take(bb);
+  l.b2 = take(ab);
+  l.c1 = take(bb);
+  l.c2 = take(ab);
+  l.e = take(bb);
+  l.g = take(bb);
+  l.dc = take(bb);
+  l.y = take(ab);
+  l.total = off;
+  return l;
+}
+
+// A <- A^{-1} for a general m x m matrix (row stride ld), in place, by
+// blocked Gauss-Jordan elimination over 64-wide pivot blocks without
+// pivoting (the ddbt* blocks are diagonally dominant).  Step k, with
+// C = A[:, k] and R = P A[k, :] for P = A_kk^{-1} (tile kernel):
+//   A_ij -= C_i R_j (i, j != k);  A_ik = -C_i P (i != k);
+//   A_kj = R_j (j != k);  A_kk = P.
+int block_inverse(double* A, int64_t ld, int m, char* ws, const DdLayout& lay, hipStream_t s) {
+  double* P = (double*)(ws + lay.p);
+  double* R = (double*)(ws + lay.r);
+  double* C = (double*)(ws + lay.c);
+  for (int k0 = 0; k0 < m; k0 += TT) {
+    const int tk = std::min(TT, m - k0), k1 = k0 + tk;
+    tile_getri_kernel<<<1, 256, 0, s>>>(A + (int64_t)k0 * ld + k0, ld, tk, P);
+    RB_CHECK(hipGetLastError());
+    if (tk < m) {
+      // C = A[:, k0:k1] (m x tk, row stride TT); R = P A[k0:k1, :] (tk x m)
+      RB_CHECK(hipMemcpy2DAsync(C, TT * sizeof(double), A + k0, ld * sizeof(double),
+                                tk * sizeof(double), m, hipMemcpyDeviceToDevice, s));
+      GemmList gr;
+      push_term(gr.add(R, m, tk, m, 0.0), term(P, TT, false, A + (int64_t)k0 * ld, ld, false, tk, 1.0));
+      int rc = run_gemms(gr, s);
+      if (rc) return rc;
+      GemmList up;
+      const int rows[2][2] = {{0, k0}, {k1, m}};
+      for (const auto& rr : rows) {
+        const int r0 = rr[0], nr = rr[1] - rr[0];
+        if (nr <= 0) continue;
+        const double* Ci = C + (int64_t)r0 * TT;
+        if (k0 > 0)
+          push_term(up.add(A + (int64_t)r0 * ld, ld, nr, k0, 1.0),
+                    term(Ci, TT, false, R, m, false, tk, -1.0));
+        if (k1 < m)
+          push_term(up.add(A + (int64_t)r0 * ld + k1, ld, nr, m - k1, 1.0),
+                    term(Ci, TT, false, R + k1, m, false, tk, -1.0));
+        push_term(up.add(A + (int64_t)r0 * ld + k0, ld, nr, tk, 0.0),
+                  term(Ci, TT, false, P, TT, false, tk, -1.0));
+      }
+      rc = run_gemms(up, s);
+      if (rc) return rc;
+      RB_CHECK(hipMemcpy2DAsync(A + (int64_t)k0 * ld, ld * sizeof(double), R, m * sizeof(double),
+                                m * sizeof(double), tk, hipMemcpyDeviceToDevice, s));
+    }
+    RB_CHECK(hipMemcpy2DAsync(A + (int64_t)k0 * ld + k0, ld * sizeof(double), P,
+                              TT * sizeof(double), tk * sizeof(double), tk,
+                              hipMemcpyDeviceToDevice, s));
+  }
+  return ROCBTA_OK;
+}
+
 // dst = src^T for one b x b block (64 x 64 tiles through LDS).
 int transpose_block(const double* src, double* dst, int64_t b, hipStream_t s) {
   const int T = (int)((b + 63) / 64);
@@ -1048,6 +1130,9 @@ size_t rocbta_workspace_bytes(int op, int64_t n, int64_t b, int64_t a, int64_t r
       return psi_layout(n, b, a).total;
     case ROCBTA_OP_POBTAS_PERMUTED:
       return 256;
+    case ROCBTA_OP_DDBTASC:
+    case ROCBTA_OP_DDBTASCI:
+      return dd_layout(b, a).total;
     case ROCBTA_OP_POBTAF_STREAMING:
       return st_layout(b, a, false).total;
     case ROCBTA_OP_POBTASI_STREAMING:
@@ -1845,6 +1930,203 @@ int rocbta_dpobtas_permuted(int64_t n, int64_t b, int64_t a, const double* D, co
       if (rc) return rc;
     }
     rc = backward_block(D + i * bb, b, bi, Bi, ldb, ri, s);
+    if (rc) return rc;
+  }
+  return ROCBTA_OK;
+}
+
+// ----------------------------------------------------- non-SPD (ddbt*)
+// Block LU Schur-complement sweep and selected inversion of diagonally
+// dominant BTA / BT matrices (no symmetry): blocked Gauss-Jordan inverses
+// of the diagonal blocks and grouped GEMMs of the updates.
+
+int rocbta_dddbtasc(int64_t n, int64_t b, int64_t a, double* D, double* Lo, double* Up,
+                    double* Rl, double* Ru, double* T, int invert_last_block, int direction,
+                    void* workspace, size_t workspace_bytes, void* stream) {
+  if (bad_shape(n, b, a) || !D || (n > 1 && (!Lo || !Up)) || (a > 0 && (!Rl || !Ru || !T)))
+    return ROCBTA_ERR_ARG;
+  if (direction != 0 && (direction != 1 || a > 0)) return ROCBTA_ERR_ARG;  // upward: BT only
+  const DdLayout lay = dd_layout(b, a);
+  if (!workspace || workspace_bytes < lay.total) return ROCBTA_ERR_WORKSPACE;
+  hipStream_t s = (hipStream_t)stream;
+  char* ws = (char*)workspace;
+  double* W = (double*)(ws + lay.w);
+  double* V = (double*)(ws + lay.v);
+  const int bi = (int)b, ai = (int)a;
+  const int64_t bb = b * b, ab = a * b;
+  int rc = ROCBTA_OK;
+  if (direction == 1) {  // ddbtsc.py:202-221
+    for (int64_t i = n - 1; i >= 1; --i) {
+      rc = block_inverse(D + i * bb, b, bi, ws, lay, s);
+      if (rc) return rc;
+      GemmList g1;
+      push_term(g1.add(W, b, bi, bi, 0.0), term(D + i * bb, b, false, Lo + (i - 1) * bb, b, false, bi, 1.0));
+      rc = run_gemms(g1, s);
+      if (rc) return rc;
+      GemmList g2;
+      push_term(g2.add(D + (i - 1) * bb, b, bi, bi, 1.0),
+                term(Up + (i - 1) * bb, b, false, W, b, false, bi, -1.0));
+      rc = run_gemms(g2, s);
+      if (rc) return rc;
+    }
+    return invert_last_block ? block_inverse(D, b, bi, ws, lay, s) : ROCBTA_OK;
+  }
+  for (int64_t i = 1; i < n; ++i) {  // ddbtasc.py:199-233, ddbtsc.py:188-195
+    double* Dp = D + (i - 1) * bb;
+    rc = block_inverse(Dp, b, bi, ws, lay, s);
+    if (rc) return rc;
+    // W = A_{i-1,i-1}^{-1} A_{i-1,i}, V = A_{i-1,i-1}^{-1} A_{i-1,t}
+    GemmList g1;
+    push_term(g1.add(W, b, bi, bi, 0.0), term(Dp, b, false, Up + (i - 1) * bb, b, false, bi, 1.0));
+    if (a > 0)
+      push_term(g1.add(V, ai, bi, ai, 0.0), term(Dp, b, false, Ru + (i - 1) * ab, ai, false, bi, 1.0));
+    rc = run_gemms(g1, s);
+    if (rc) return rc;
+    GemmList g2;
+    push_term(g2.add(D + i * bb, b, bi, bi, 1.0), term(Lo + (i - 1) * bb, b, false, W, b, false, bi, -1.0));
+    if (a > 0) {
+      push_term(g2.add(Rl + i * ab, b, ai, bi, 1.0), term(Rl + (i - 1) * ab, b, false, W, b, false, bi, -1.0));
+      push_term(g2.add(Ru + i * ab, ai, bi, ai, 1.0), term(Lo + (i - 1) * bb, b, false, V, ai, false, bi, -1.0));
+      push_term(g2.add(T, a, ai, ai, 1.0), term(Rl + (i - 1) * ab, b, false, V, ai, false, bi, -1.0));
+    }
+    rc = run_gemms(g2, s);
+    if (rc) return rc;
+  }
+  if (!invert_last_block) return ROCBTA_OK;
+  double* Dl = D + (n - 1) * bb;  // ddbtasc.py:235-241
+  rc = block_inverse(Dl, b, bi, ws, lay, s);
+  if (rc || a == 0) return rc;
+  GemmList g1;
+  push_term(g1.add(V, ai, bi, ai, 0.0), term(Dl, b, false, Ru + (n - 1) * ab, ai, false, bi, 1.0));
+  rc = run_gemms(g1, s);
+  if (rc) return rc;
+  GemmList g2;
+  push_term(g2.add(T, a, ai, ai, 1.0), term(Rl + (n - 1) * ab, b, false, V, ai, false, bi, -1.0));
+  rc = run_gemms(g2, s);
+  if (rc) return rc;
+  return block_inverse(T, a, ai, ws, lay, s);
+}
+
+int rocbta_dddbtasci(int64_t n, int64_t b, int64_t a, double* D, double* Lo, double* Up,
+                     double* Rl, double* Ru, double* T, int invert_last_block, int direction,
+                     void* workspace, size_t workspace_bytes, void* stream) {
+  if (bad_shape(n, b, a) || !D || (n > 1 && (!Lo || !Up)) || (a > 0 && (!Rl || !Ru || !T)))
+    return ROCBTA_ERR_ARG;
+  if (direction != 0 && (direction != 1 || a > 0)) return ROCBTA_ERR_ARG;
+  const DdLayout lay = dd_layout(b, a);
+  if (!workspace || workspace_bytes < lay.total) return ROCBTA_ERR_WORKSPACE;
+  hipStream_t s = (hipStream_t)stream;
+  char* ws = (char*)workspace;
+  double* B1 = (double*)(ws + lay.b1);
+  double* B2 = (double*)(ws + lay.b2);
+  double* C1 = (double*)(ws + lay.c1);
+  double* C2 = (double*)(ws + lay.c2);
+  double* E = (double*)(ws + lay.e);
+  double* Gm = (double*)(ws + lay.g);
+  double* Dc = (double*)(ws + lay.dc);
+  double* Y = (double*)(ws + lay.y);
+  const int bi = (int)b, ai = (int)a;
+  const int64_t bb = b * b, ab = a * b;
+  int rc = ROCBTA_OK;
+  if (direction == 1) {  // ddbtsci.py:208-237
+    for (int64_t i = 1; i < n; ++i) {
+      double* Di = D + i * bb;
+      double* Dm = D + (i - 1) * bb;
+      double* Lm = Lo + (i - 1) * bb;
+      double* Um = Up + (i - 1) * bb;
+      RB_CHECK(hipMemcpyAsync(Dc, Di, bb * sizeof(double), hipMemcpyDeviceToDevice, s));
+      // B1 = A_{i,i-1} X_{i-1,i-1}; C1 = X_{i-1,i-1} A_{i-1,i}
+      GemmList g1;
+      push_term(g1.add(B1, b, bi, bi, 0.0), term(Lm, b, false, Dm, b, false, bi, 1.0));
+      push_term(g1.add(C1, b, bi, bi, 0.0), term(Dm, b, false, Um, b, false, bi, 1.0));
+      rc = run_gemms(g1, s);
+      if (rc) return rc;
+      // E = B1 A_{i-1,i} (old), then the new off-diagonal blocks
+      GemmList g2;
+      push_term(g2.add(E, b, bi, bi, 0.0), term(B1, b, false, Um, b, false, bi, 1.0));
+      push_term(g2.add(Lm, b, bi, bi, 0.0), term(Dc, b, false, B1, b, false, bi, -1.0));
+      rc = run_gemms(g2, s);
+      if (rc) return rc;
+      GemmList g3;
+      push_term(g3.add(Um, b, bi, bi, 0.0), term(C1, b, false, Dc, b, false, bi, -1.0));
+      push_term(g3.add(Gm, b, bi, bi, 0.0), term(Dc, b, false, E, b, false, bi, 1.0));
+      rc = run_gemms(g3, s);
+      if (rc) return rc;
+      GemmList g4;
+      push_term(g4.add(Di, b, bi, bi, 1.0), term(Gm, b, false, Dc, b, false, bi, 1.0));
+      rc = run_gemms(g4, s);
+      if (rc) return rc;
+    }
+    return ROCBTA_OK;
+  }
+  if (invert_last_block && a > 0) {  // ddbtasci.py:207-219
+    double* Dl = D + (n - 1) * bb;
+    double* Rll = Rl + (n - 1) * ab;
+    double* Rul = Ru + (n - 1) * ab;
+    GemmList g1;
+    push_term(g1.add(B2, ai, bi, ai, 0.0), term(Dl, b, false, Rul, ai, false, bi, 1.0));
+    push_term(g1.add(Y, b, ai, bi, 0.0), term(Rll, b, false, Dl, b, false, bi, 1.0));
+    rc = run_gemms(g1, s);
+    if (rc) return rc;
+    GemmList g2;
+    push_term(g2.add(Rll, b, ai, bi, 0.0), term(T, a, false, Y, b, false, ai, -1.0));
+    push_term(g2.add(Rul, ai, bi, ai, 0.0), term(B2, ai, false, T, a, false, ai, -1.0));
+    rc = run_gemms(g2, s);
+    if (rc) return rc;
+    GemmList g3;
+    push_term(g3.add(Dl, b, bi, bi, 1.0), term(B2, ai, false, Rll, b, false, ai, -1.0));
+    rc = run_gemms(g3, s);
+    if (rc) return rc;
+  }
+  for (int64_t i = n - 2; i >= 0; --i) {  // ddbtasci.py:221-256, ddbtsci.py:187-205
+    double* Di = D + i * bb;
+    double* D1 = D + (i + 1) * bb;
+    double* Li = Lo + i * bb;
+    double* Ui = Up + i * bb;
+    double* Rli = Rl + i * ab;
+    double* Rui = Ru + i * ab;
+    double* Rl1 = Rl + (i + 1) * ab;
+    double* Ru1 = Ru + (i + 1) * ab;
+    RB_CHECK(hipMemcpyAsync(Dc, Di, bb * sizeof(double), hipMemcpyDeviceToDevice, s));
+    GemmList g1;
+    GemmProblem& pb1 = g1.add(B1, b, bi, bi, 0.0);
+    push_term(pb1, term(Ui, b, false, D1, b, false, bi, 1.0));
+    GemmProblem& pc1 = g1.add(C1, b, bi, bi, 0.0);
+    push_term(pc1, term(D1, b, false, Li, b, false, bi, 1.0));
+    if (a > 0) {
+      push_term(pb1, term(Rui, ai, false, Rl1, b, false, ai, 1.0));
+      push_term(pc1, term(Ru1, ai, false, Rli, b, false, ai, 1.0));
+      GemmProblem& pb2 = g1.add(B2, ai, bi, ai, 0.0);
+      push_term(pb2, term(Ui, b, false, Ru1, ai, false, bi, 1.0));
+      push_term(pb2, term(Rui, ai, false, T, a, false, ai, 1.0));
+      GemmProblem& pc2 = g1.add(C2, b, ai, bi, 0.0);
+      push_term(pc2, term(Rl1, b, false, Li, b, false, bi, 1.0));
+      push_term(pc2, term(T, a, false, Rli, b, false, ai, 1.0));
+    }
+    rc = run_gemms(g1, s);
+    if (rc) return rc;
+    // E = B1 A_{i+1,i} + B2 A_{t,i} (old blocks); X_{i,i+1}, X_{i,t}
+    GemmList g2;
+    GemmProblem& pe = g2.add(E, b, bi, bi, 0.0);
+    push_term(pe, term(B1, b, false, Li, b, false, bi, 1.0));
+    push_term(g2.add(Ui, b, bi, bi, 0.0), term(Dc, b, false, B1, b, false, bi, -1.0));
+    if (a > 0) {
+      push_term(pe, term(B2, ai, false, Rli, b, false, ai, 1.0));
+      push_term(g2.add(Rui, ai, bi, ai, 0.0), term(Dc, b, false, B2, ai, false, bi, -1.0));
+    }
+    rc = run_gemms(g2, s);
+    if (rc) return rc;
+    // X_{i+1,i}, X_{t,i}, and Gm = X_ii E
+    GemmList g3;
+    push_term(g3.add(Li, b, bi, bi, 0.0), term(C1, b, false, Dc, b, false, bi, -1.0));
+    if (a > 0) push_term(g3.add(Rli, b, ai, bi, 0.0), term(C2, b, false, Dc, b, false, bi, -1.0));
+    push_term(g3.add(Gm, b, bi, bi, 0.0), term(Dc, b, false, E, b, false, bi, 1.0));
+    rc = run_gemms(g3, s);
+    if (rc) return rc;
+    // X_ii = D_i + D_i E D_i
+    GemmList g4;
+    push_term(g4.add(Di, b, bi, bi, 1.0), term(Gm, b, false, Dc, b, false, bi, 1.0));
+    rc = run_gemms(g4, s);
     if (rc) return rc;
   }
   return ROCBTA_OK;

@@ -128,3 +128,19 @@ def test_streaming_workspace_and_argument_errors(lib):
                                          None) == _lib.ROCBTA_ERR_ARG
     assert lib.rocbta_dpobtasi_streaming(3, 4, 1, fake, fake, fake, fake, 1, fake, 8,
                                          None) == _lib.ROCBTA_ERR_WORKSPACE
+
+
+def test_ddbt_workspace_and_argument_errors(lib):
+    assert lib.rocbta_workspace_bytes(_lib.OP_DDBTASC, 4, 256, 32, 0) >= 7 * 256 * 256 * 8
+    assert lib.rocbta_workspace_bytes(_lib.OP_DDBTASCI, 4, 256, 32, 0) > 0
+    fake = ctypes.c_void_p(16)
+    big = 1 << 24
+    # upward sweeps exist for the arrowless form only
+    assert lib.rocbta_dddbtasc(3, 4, 2, fake, fake, fake, fake, fake, fake, 1, 1, fake, big,
+                               None) == _lib.ROCBTA_ERR_ARG
+    assert lib.rocbta_dddbtasci(3, 4, 0, fake, fake, fake, None, None, None, 1, 2, fake, big,
+                                None) == _lib.ROCBTA_ERR_ARG
+    assert lib.rocbta_dddbtasc(3, 4, 2, fake, fake, None, fake, fake, fake, 1, 0, fake, big,
+                               None) == _lib.ROCBTA_ERR_ARG
+    assert lib.rocbta_dddbtasci(3, 4, 2, fake, fake, fake, fake, fake, fake, 1, 0, fake, 8,
+                                None) == _lib.ROCBTA_ERR_WORKSPACE

@@ -214,3 +214,44 @@ PERMUTED_SOLVE_MUTANTS = [
 def test_permuted_solve_mutants_are_caught(old, new):
     bad = mutant(serinv_np.pobtas_permuted, old, new)
     assert permuted_errs(permuted_rt(sv=bad))[2] > 1e-6
+
+
+# Non-SPD family: mutants of the sweep and of the inversion must change the
+# selected inverse against the dense inverse of a non-symmetric matrix.
+def ddbt_err(sc=None, sci=None):
+    b, a, n = 3, 2, 4
+    A = generators.dd_bta(b, a, n, np.random.default_rng(77))
+    arrs = dense.bta_dense_to_arrays_full(A, b, a, n)
+    (sc or serinv_np.ddbtasc)(*arrs)
+    (sci or serinv_np.ddbtasci)(*arrs)
+    ref = dense.bta_dense_to_arrays_full(np.linalg.inv(A), b, a, n)
+    return max(np.max(np.abs(x - y)) / np.max(np.abs(y)) for x, y in zip(arrs, ref))
+
+
+def test_unmutated_ddbt_oracle_passes():
+    assert ddbt_err() < TOL
+
+
+DDBT_SC_MUTANTS = [
+    ("Ru[i] = Ru[i] - Lo[i - 1] @ D[i - 1] @ Ru[i - 1]", "Ru[i] = Ru[i] - Up[i - 1] @ D[i - 1] @ Ru[i - 1]"),
+    ("T[:, :] = T - Rl[i - 1] @ D[i - 1] @ Ru[i - 1]", "T[:, :] = T"),
+    ("D[i] = D[i] - Lo[i - 1] @ D[i - 1] @ Up[i - 1]", "D[i] = D[i] - Up[i - 1] @ D[i - 1] @ Lo[i - 1]"),
+]
+
+
+@pytest.mark.parametrize("old,new", DDBT_SC_MUTANTS)
+def test_ddbtasc_mutants_are_caught(old, new):
+    assert ddbt_err(sc=mutant(serinv_np.ddbtasc, old, new)) > 1e-6
+
+
+DDBT_SCI_MUTANTS = [
+    ("C1 = D[i + 1] @ Lo[i] + Ru[i + 1] @ Rl[i]", "C1 = D[i + 1] @ Lo[i]"),
+    ("Rl[i] = -C2 @ D[i]", "Rl[i] = C2 @ D[i]"),
+    ("D[i] = D[i] + D[i] @ (B1 @ D1 + B2 @ D2) @ D[i]", "D[i] = D[i] + D[i] @ (B1 @ D1) @ D[i]"),
+    ("B2 = Up[i] @ Ru[i + 1] + Ru[i] @ T", "B2 = Up[i] @ Ru[i + 1] + Ru[i] @ T.T"),
+]
+
+
+@pytest.mark.parametrize("old,new", DDBT_SCI_MUTANTS)
+def test_ddbtasci_mutants_are_caught(old, new):
+    assert ddbt_err(sci=mutant(serinv_np.ddbtasci, old, new)) > 1e-6
