@@ -279,6 +279,24 @@ int rocbta_dddbtasci(int64_t n, int64_t b, int64_t a, double* D, double* Lo, dou
                      double* Rl, double* Ru, double* T, int invert_last_block, int direction,
                      void* workspace, size_t workspace_bytes, void* stream);
 
+/* The A X A^T = B forms ("quadratic"): the same sweeps with the blocks of a
+ * symmetric B (BD, BLo, BUp, BRl, BRu, BT, same shapes as A's) carried
+ * along; after both sweeps A's arrays hold A^{-1} and B's A^{-1} B A^{-T}
+ * on the pattern.  Downward only (a == 0 gives the BT forms).  Replace
+ * _ddbtasc_quadratic (ddbtasc.py:335-475) / _ddbtsc_quadratic (ddbtsc.py:
+ * 271-311) and _ddbtasci_quadratic (ddbtasci.py:343-576) / _ddbtsci_quadratic
+ * (ddbtsci.py:299-369).  Workspace: ROCBTA_OP_DDBTASC. */
+int rocbta_dddbtasc_quadratic(int64_t n, int64_t b, int64_t a, double* D, double* Lo, double* Up,
+                              double* Rl, double* Ru, double* T, double* BD, double* BLo,
+                              double* BUp, double* BRl, double* BRu, double* BT,
+                              int invert_last_block, void* workspace, size_t workspace_bytes,
+                              void* stream);
+int rocbta_dddbtasci_quadratic(int64_t n, int64_t b, int64_t a, double* D, double* Lo,
+                               double* Up, double* Rl, double* Ru, double* T, double* BD,
+                               double* BLo, double* BUp, double* BRl, double* BRu, double* BT,
+                               int invert_last_block, void* workspace, size_t workspace_bytes,
+                               void* stream);
+
 /* -------------------------------------------------------------- streaming
  * Host <-> device streaming forms for systems whose blocks do not fit in
  * device memory: D, L, R, T are HOST pointers (page-locked memory, e.g.

@@ -381,3 +381,79 @@ def ddbtsci(D, Lo, Up, direction="downward"):
             D[i] = D[i] - Lo[i - 1] @ tu @ D[i]
     else:
         raise ValueError("Optional keyword argument `direction` must be either 'downward' or 'upward'")
+
+
+def ddbtasc_quadratic(D, Lo, Up, Rl, Ru, T, BD, BLo, BUp, BRl, BRu, BT, invert_last_block=True):
+    """Schur-complement sweep for A X A^T = B, src/serinv/algs/ddbtasc.py:335-475:
+    the A sweep of ddbtasc, with B's blocks carried through the same
+    elimination (B[i-1] <- A_ii^{-1} B[i-1] A_ii^{-T} and the updates of the
+    next block row)."""
+    n = D.shape[0]
+    for i in range(1, n):  # :351
+        D[i - 1] = np.linalg.inv(D[i - 1])  # :353
+        D[i] = D[i] - Lo[i - 1] @ D[i - 1] @ Up[i - 1]  # :356-361
+        Rl[i] = Rl[i] - Rl[i - 1] @ D[i - 1] @ Up[i - 1]  # :364-369
+        Ru[i] = Ru[i] - Lo[i - 1] @ D[i - 1] @ Ru[i - 1]  # :372-377
+        T[:, :] = T - Rl[i - 1] @ D[i - 1] @ Ru[i - 1]  # :380-385
+        Di, L, R = D[i - 1], Lo[i - 1], Rl[i - 1]
+        BD[i - 1] = Di @ BD[i - 1] @ Di.T  # :389-393
+        BD[i] = BD[i] + L @ BD[i - 1] @ L.T - BLo[i - 1] @ Di.T @ L.T - L @ Di @ BUp[i - 1]  # :395-406
+        BRu[i] = BRu[i] + L @ BD[i - 1] @ R.T - BLo[i - 1] @ Di.T @ R.T - L @ Di @ BRu[i - 1]  # :408-419
+        BRl[i] = BRl[i] + R @ BD[i - 1] @ L.T - BRl[i - 1] @ Di.T @ L.T - R @ Di @ BUp[i - 1]  # :421-432
+        BT[:, :] = BT + R @ BD[i - 1] @ R.T - BRl[i - 1] @ Di.T @ R.T - R @ Di @ BRu[i - 1]  # :434-445
+    if invert_last_block:  # :447-475
+        D[-1] = np.linalg.inv(D[-1])
+        BD[-1] = D[-1] @ BD[-1] @ D[-1].T
+        T[:, :] = np.linalg.inv(T - Rl[-1] @ D[-1] @ Ru[-1])
+        BT[:, :] = T @ (BT + Rl[-1] @ BD[-1] @ Rl[-1].T - BRl[-1] @ D[-1].T @ Rl[-1].T
+                        - Rl[-1] @ D[-1] @ BRu[-1]) @ T.T
+
+
+def ddbtasci_quadratic(D, Lo, Up, Rl, Ru, T, BD, BLo, BUp, BRl, BRu, BT, invert_last_block=True):
+    """Selected inversion for A X A^T = B from ddbtasc_quadratic's output,
+    src/serinv/algs/ddbtasci.py:343-576: on exit D.. hold the blocks of
+    A^{-1} and BD.. those of A^{-1} B A^{-T}.  Like the reference, for a
+    symmetric B (it uses B_ii^T for B_ii, :540-544)."""
+    n = D.shape[0]
+    if invert_last_block:  # :377-430
+        B2 = D[-1] @ Ru[-1]
+        C2 = Ru[-1].T @ D[-1].T
+        D2 = T @ Rl[-1] @ BD[-1]
+        t13 = BRu[-1].copy()
+        t31 = BRl[-1].copy()
+        BRu[-1] = -B2 @ BT - BD[-1] @ Rl[-1].T @ T.T + D[-1] @ BRu[-1] @ T.T
+        BRl[-1] = -BT @ C2 - D2 + T @ BRl[-1] @ D[-1].T
+        BD[-1] = (BD[-1] + B2 @ BT @ C2 + B2 @ D2 + BD[-1].T @ Rl[-1].T @ T.T @ C2
+                  - B2 @ T @ t31 @ D[-1].T - D[-1] @ t13 @ T.T @ C2)
+        Rl[-1] = -T @ Rl[-1] @ D[-1]
+        Ru[-1] = -B2 @ T
+        D[-1] = D[-1] - B2 @ Rl[-1]
+    for i in range(n - 2, -1, -1):  # :432
+        B1 = Up[i] @ D[i + 1] + Ru[i] @ Rl[i + 1]
+        B2 = Up[i] @ Ru[i + 1] + Ru[i] @ T
+        C1 = D[i + 1] @ Lo[i] + Ru[i + 1] @ Rl[i]
+        C2 = Rl[i + 1] @ Lo[i] + T @ Rl[i]
+        t12, t13, t21, t31 = BUp[i].copy(), BRu[i].copy(), BLo[i].copy(), BRl[i].copy()  # :453-456
+        BUp[i] = (-D[i] @ (Up[i] @ BD[i + 1] + Ru[i] @ BRl[i + 1]) - BD[i] @ C1.T
+                  + D[i] @ (BUp[i] @ D[i + 1].T + BRu[i] @ Ru[i + 1].T))  # :458-469
+        BRu[i] = (-D[i] @ (Up[i] @ BRu[i + 1] + Ru[i] @ BT) - BD[i] @ C2.T
+                  + D[i] @ (t12 @ Rl[i + 1].T + BRu[i] @ T.T))  # :470-483
+        BLo[i] = (-(BD[i + 1] @ Up[i].T + BRu[i + 1] @ Ru[i].T) @ D[i].T - C1 @ BD[i]
+                  + (D[i + 1] @ BLo[i] + Ru[i + 1] @ BRl[i]) @ D[i].T)  # :485-496
+        BRl[i] = (-(BRl[i + 1] @ Up[i].T + BT @ Ru[i].T) @ D[i].T - C2 @ BD[i]
+                  + (Rl[i + 1] @ t21 + T @ BRl[i]) @ D[i].T)  # :497-508
+        BD[i] = (BD[i]
+                 + D[i] @ ((Up[i] @ BD[i + 1] + Ru[i] @ BRl[i + 1]) @ Up[i].T
+                           + (Up[i] @ BRu[i + 1] + Ru[i] @ BT) @ Ru[i].T) @ D[i].T
+                 + D[i] @ (B1 @ Lo[i] + B2 @ Rl[i]) @ BD[i]
+                 + BD[i].T @ (C1.T @ Up[i].T + C2.T @ Ru[i].T) @ D[i].T
+                 - D[i] @ (B1 @ t21 + B2 @ t31) @ D[i].T
+                 - D[i] @ ((t12 @ D[i + 1].T + t13 @ Ru[i + 1].T) @ Up[i].T
+                           + (t12 @ Rl[i + 1].T + t13 @ T.T) @ Ru[i].T) @ D[i].T)  # :510-559
+        Up[i] = -D[i] @ B1  # :562
+        Ru[i] = -D[i] @ B2
+        D1 = Lo[i].copy()
+        D2 = Rl[i].copy()
+        Lo[i] = -C1 @ D[i]
+        Rl[i] = -C2 @ D[i]
+        D[i] = D[i] + D[i] @ (B1 @ D1 + B2 @ D2) @ D[i]  # :571-576

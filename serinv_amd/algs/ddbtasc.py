@@ -6,8 +6,12 @@ and :187-241 (the "AX=I" sweep); ddbtasci.py:9-185 and :187-256;
 ddbtsc.py:9-221 and ddbtsci.py:9-237 (the arrowless forms, downward and
 upward).  They run as rocbta_dddbtasc / rocbta_dddbtasci: blocked
 Gauss-Jordan inverses of the diagonally dominant diagonal blocks (no
-pivoting) and grouped GEMMs.  The rhs= / quadratic= ("AXA^T=B") and
-buffers= (permuted) forms raise NotImplementedError.
+pivoting) and grouped GEMMs.  rhs= with quadratic=True ("AXA^T=B",
+ddbtasc.py:335-475, ddbtasci.py:343-576; downward) runs as
+rocbta_dddbtasc_quadratic / rocbta_dddbtasci_quadratic.  rhs= without
+quadratic (the reference's "AX=B" branch is an empty placeholder,
+ddbtasc.py:183-185), the permuted buffers= forms and the upward quadratic
+BT forms raise NotImplementedError.
 """
 
 import torch
@@ -45,12 +49,32 @@ def _shapes(D, Lo, Up, Rl, Ru, T):
     return n, b, a, D.device
 
 
-def _reject_forms(kwargs, what):
-    # ddbtasc.py:74-185: rhs (AX=B / AXA^T=B) and buffers (permuted) forms
-    if kwargs.get("rhs", None) is not None:
-        raise NotImplementedError(f"{what}: the rhs= (AX=B / AXA^T=B) forms are not provided")
+_B_KEYS = ("B_diagonal_blocks", "B_lower_diagonal_blocks", "B_upper_diagonal_blocks",
+           "B_lower_arrow_blocks", "B_upper_arrow_blocks", "B_arrow_tip_block")
+
+
+def _rhs_arrays(kwargs, what, arrow):
+    """The B blocks of the A X A^T = B form, or None for A X = I
+    (ddbtasc.py:74-126 for the checks and their messages)."""
     if kwargs.get("buffers", None) is not None:
         raise NotImplementedError(f"{what}: the permuted (buffers=) form is not provided")
+    rhs = kwargs.get("rhs", None)
+    if rhs is None:
+        return None
+    keys = _B_KEYS if arrow else _B_KEYS[:3]
+    arrs = [rhs.get(k, None) for k in keys]
+    if any(x is None for x in arrs):
+        raise ValueError("rhs does not contain the correct arrays")
+    if not kwargs.get("quadratic", False):
+        raise NotImplementedError(f"{what}: the AX=B form (rhs= without quadratic=True) is not "
+                                  "provided")
+    return arrs
+
+
+def _check_like(B, A, name):
+    _tensor(B, name)
+    if tuple(B.shape) != tuple(A.shape) or B.device != A.device:
+        raise ValueError(f"{name} must match the shape {tuple(A.shape)} and device of A's block")
 
 
 def _direction(kwargs):
@@ -63,9 +87,22 @@ def _direction(kwargs):
     raise ValueError("Optional keyword argument `direction` must be either 'downward' or 'upward'")
 
 
-def _run(fn_name, op, D, Lo, Up, Rl, Ru, T, n, b, a, dev, invert_last_block, direction, what):
+def _run(fn_name, op, D, Lo, Up, Rl, Ru, T, n, b, a, dev, invert_last_block, direction, what,
+         B=None):
     lib = _lib.load()
     ws, nbytes = workspace(op, n, b, a, 0, dev)
+    if B is not None:  # A X A^T = B: downward only
+        if direction != 0:
+            raise NotImplementedError(f"{what}: the upward A X A^T = B form is not provided")
+        for x, y, nm in zip(B, (D, Lo, Up, Rl, Ru, T), _B_KEYS):
+            if y is not None:
+                _check_like(x, y, nm)
+        Bp = [ptr(x) for x in B] + [None] * (6 - len(B))
+        rc = getattr(lib, fn_name + "_quadratic")(
+            n, b, a, ptr(D), ptr(Lo), ptr(Up), ptr(Rl), ptr(Ru), ptr(T), *Bp,
+            int(invert_last_block), ws.data_ptr(), nbytes, stream_of(dev))
+        _lib.check(rc, what)
+        return
     rc = getattr(lib, fn_name)(n, b, a, ptr(D), ptr(Lo), ptr(Up), ptr(Rl), ptr(Ru), ptr(T),
                                int(invert_last_block), int(direction), ws.data_ptr(), nbytes,
                                stream_of(dev))
@@ -80,12 +117,12 @@ def ddbtasc(A_diagonal_blocks, A_lower_diagonal_blocks, A_upper_diagonal_blocks,
     the arrow blocks and the tip their Schur-updated values (the tip
     inverted with invert_last_block=True, the default)."""
     check_kwargs(kwargs, {"rhs", "quadratic", "buffers", "invert_last_block"})
-    _reject_forms(kwargs, "ddbtasc")
+    B = _rhs_arrays(kwargs, "ddbtasc", True)
     arrs = (A_diagonal_blocks, A_lower_diagonal_blocks, A_upper_diagonal_blocks,
             A_lower_arrow_blocks, A_upper_arrow_blocks, A_arrow_tip_block)
     n, b, a, dev = _shapes(*arrs)
     _run("rocbta_dddbtasc", _lib.OP_DDBTASC, *arrs, n, b, a, dev,
-         bool(kwargs.get("invert_last_block", True)), 0, "ddbtasc")
+         bool(kwargs.get("invert_last_block", True)), 0, "ddbtasc", B)
 
 
 def ddbtasci(A_diagonal_blocks, A_lower_diagonal_blocks, A_upper_diagonal_blocks,
@@ -93,32 +130,32 @@ def ddbtasci(A_diagonal_blocks, A_lower_diagonal_blocks, A_upper_diagonal_blocks
     """Selected inversion from the ddbtasc output, in place: the blocks of
     A^{-1} on the BTA pattern."""
     check_kwargs(kwargs, {"rhs", "quadratic", "buffers", "invert_last_block"})
-    _reject_forms(kwargs, "ddbtasci")
+    B = _rhs_arrays(kwargs, "ddbtasci", True)
     arrs = (A_diagonal_blocks, A_lower_diagonal_blocks, A_upper_diagonal_blocks,
             A_lower_arrow_blocks, A_upper_arrow_blocks, A_arrow_tip_block)
     n, b, a, dev = _shapes(*arrs)
     _run("rocbta_dddbtasci", _lib.OP_DDBTASCI, *arrs, n, b, a, dev,
-         bool(kwargs.get("invert_last_block", True)), 0, "ddbtasci")
+         bool(kwargs.get("invert_last_block", True)), 0, "ddbtasci", B)
 
 
 def ddbtsc(A_diagonal_blocks, A_lower_diagonal_blocks, A_upper_diagonal_blocks, **kwargs):
     """Schur-complement sweep of a diagonally dominant BT matrix, in place;
     direction "downward" (default) or "upward"."""
     check_kwargs(kwargs, {"rhs", "quadratic", "buffers", "invert_last_block", "direction"})
-    _reject_forms(kwargs, "ddbtsc")
+    B = _rhs_arrays(kwargs, "ddbtsc", False)
     direction = _direction(kwargs)
     D, Lo, Up = A_diagonal_blocks, A_lower_diagonal_blocks, A_upper_diagonal_blocks
     n, b, a, dev = _shapes(D, Lo, Up, None, None, None)
     _run("rocbta_dddbtasc", _lib.OP_DDBTASC, D, Lo, Up, None, None, None, n, b, 0, dev,
-         bool(kwargs.get("invert_last_block", True)), direction, "ddbtsc")
+         bool(kwargs.get("invert_last_block", True)), direction, "ddbtsc", B)
 
 
 def ddbtsci(A_diagonal_blocks, A_lower_diagonal_blocks, A_upper_diagonal_blocks, **kwargs):
     """Selected inversion from the ddbtsc output (same direction), in place."""
     check_kwargs(kwargs, {"rhs", "quadratic", "buffers", "direction"})
-    _reject_forms(kwargs, "ddbtsci")
+    B = _rhs_arrays(kwargs, "ddbtsci", False)
     direction = _direction(kwargs)
     D, Lo, Up = A_diagonal_blocks, A_lower_diagonal_blocks, A_upper_diagonal_blocks
     n, b, a, dev = _shapes(D, Lo, Up, None, None, None)
     _run("rocbta_dddbtasci", _lib.OP_DDBTASCI, D, Lo, Up, None, None, None, n, b, 0, dev, True,
-         direction, "ddbtsci")
+         direction, "ddbtsci", B)

@@ -1005,7 +1005,10 @@ StLayout st_layout(int64_t b, int64_t a, bool selinv) {
 // Workspace of the non-SPD sweeps: the blocked inverse's pivot-tile
 // inverse P and its row / column panels, and the products of one step.
 struct DdLayout {
-  size_t p, r, c, w, v, b1, b2, c1, c2, e, g, dc, y, total;
+  size_t p, r, c, w, v, b1, b2, c1, c2, e, g, dc, y;
+  // quadratic (A X A^T = B) forms
+  size_t bdc, df1, df2, dk1, dk2, sm, mt, q1, q2, q3, z, p1, p2, m1, m2, v1, zt, p3, d2, bbt, y2,
+      total;
 };
 DdLayout dd_layout(int64_t b, int64_t a) {
   DdLayout l{};
@@ -1030,6 +1033,28 @@ DdLayout dd_layout(int64_t b, int64_t a) {
   l.g = take(bb);
   l.dc = take(bb);
   l.y = take(ab);
+  const size_t aa = (size_t)a1 * a1 * sizeof(double);
+  l.bdc = take(bb);
+  l.df1 = take(bb);
+  l.df2 = take(ab);
+  l.dk1 = take(bb);
+  l.dk2 = take(ab);
+  l.sm = take(bb);
+  l.mt = take(bb);
+  l.q1 = take(bb);
+  l.q2 = take(bb);
+  l.q3 = take(bb);
+  l.z = take(bb);
+  l.p1 = take(bb);
+  l.p2 = take(ab);
+  l.m1 = take(bb);
+  l.m2 = take(ab);
+  l.v1 = take(ab);
+  l.zt = take(ab);
+  l.p3 = take(ab);
+  l.d2 = take(ab);
+  l.bbt = take(std::max(ab, aa));
+  l.y2 = take(ab);
   l.total = off;
   return l;
 }
@@ -2128,6 +2153,269 @@ int rocbta_dddbtasci(int64_t n, int64_t b, int64_t a, double* D, double* Lo, dou
     push_term(g4.add(Di, b, bi, bi, 1.0), term(Gm, b, false, Dc, b, false, bi, 1.0));
     rc = run_gemms(g4, s);
     if (rc) return rc;
+  }
+  return ROCBTA_OK;
+}
+
+// A X A^T = B forms: the same sweeps with B's blocks carried along.
+namespace {
+struct Gq {  // one grouped launch, built problem by problem
+  GemmList l;
+  GemmProblem* cur = nullptr;
+  Gq& p(double* C, int64_t ldc, int64_t M, int64_t N, double beta) {
+    cur = &l.add(C, ldc, (int)M, (int)N, beta);
+    return *this;
+  }
+  // alpha * op(A) op(B), K = k
+  Gq& t(const double* A, int64_t lda, bool tA, const double* B, int64_t ldb, bool tB, int64_t k,
+        double alpha) {
+    push_term(*cur, term(A, lda, tA, B, ldb, tB, (int)k, alpha));
+    return *this;
+  }
+  int run(hipStream_t s) { return run_gemms(l, s); }
+};
+}  // namespace
+
+int rocbta_dddbtasc_quadratic(int64_t n, int64_t b, int64_t a, double* D, double* Lo, double* Up,
+                              double* Rl, double* Ru, double* T, double* BD, double* BLo,
+                              double* BUp, double* BRl, double* BRu, double* BT,
+                              int invert_last_block, void* workspace, size_t workspace_bytes,
+                              void* stream) {
+  if (bad_shape(n, b, a) || !D || !BD || (n > 1 && (!Lo || !Up || !BLo || !BUp)) ||
+      (a > 0 && (!Rl || !Ru || !T || !BRl || !BRu || !BT)))
+    return ROCBTA_ERR_ARG;
+  const DdLayout lay = dd_layout(b, a);
+  if (!workspace || workspace_bytes < lay.total) return ROCBTA_ERR_WORKSPACE;
+  hipStream_t s = (hipStream_t)stream;
+  char* ws = (char*)workspace;
+  auto P = [&](size_t off) { return (double*)(ws + off); };
+  double *W = P(lay.w), *V = P(lay.v), *Z = P(lay.z), *P1 = P(lay.p1), *P2 = P(lay.p2);
+  double *M1 = P(lay.m1), *M2 = P(lay.m2), *Y1 = P(lay.bbt);
+  const int64_t bb = b * b, ab = a * b;
+  int rc = ROCBTA_OK;
+  for (int64_t i = 1; i < n; ++i) {  // ddbtasc.py:351-445
+    double* Di = D + (i - 1) * bb;
+    const double* L = Lo + (i - 1) * bb;
+    const double* U = Up + (i - 1) * bb;
+    const double* R = Rl + (i - 1) * ab;
+    const double* Rur = Ru + (i - 1) * ab;
+    rc = block_inverse(Di, b, (int)b, ws, lay, s);
+    if (rc) return rc;
+    Gq g1;
+    g1.p(W, b, b, b, 0.0).t(Di, b, false, U, b, false, b, 1.0);
+    g1.p(Z, b, b, b, 0.0).t(Di, b, false, BD + (i - 1) * bb, b, false, b, 1.0);
+    g1.p(P1, b, b, b, 0.0).t(L, b, false, Di, b, false, b, 1.0);
+    if (a > 0) {
+      g1.p(V, a, b, a, 0.0).t(Di, b, false, Rur, a, false, b, 1.0);
+      g1.p(P2, b, a, b, 0.0).t(R, b, false, Di, b, false, b, 1.0);
+    }
+    if ((rc = g1.run(s))) return rc;
+    Gq g2;  // A's Schur updates; B_{i-1} <- A_{i-1}^{-1} B_{i-1} A_{i-1}^{-T}
+    g2.p(D + i * bb, b, b, b, 1.0).t(L, b, false, W, b, false, b, -1.0);
+    g2.p(BD + (i - 1) * bb, b, b, b, 0.0).t(Z, b, false, Di, b, true, b, 1.0);
+    if (a > 0) {
+      g2.p(Rl + i * ab, b, a, b, 1.0).t(R, b, false, W, b, false, b, -1.0);
+      g2.p(Ru + i * ab, a, b, a, 1.0).t(L, b, false, V, a, false, b, -1.0);
+      g2.p(T, a, a, a, 1.0).t(R, b, false, V, a, false, b, -1.0);
+    }
+    if ((rc = g2.run(s))) return rc;
+    const double* Bp = BD + (i - 1) * bb;
+    Gq g3;
+    g3.p(M1, b, b, b, 0.0).t(L, b, false, Bp, b, false, b, 1.0);
+    if (a > 0) g3.p(M2, b, a, b, 0.0).t(R, b, false, Bp, b, false, b, 1.0);
+    if ((rc = g3.run(s))) return rc;
+    const double* BLp = BLo + (i - 1) * bb;
+    const double* BUp_ = BUp + (i - 1) * bb;
+    const double* BRlp = BRl + (i - 1) * ab;
+    const double* BRup = BRu + (i - 1) * ab;
+    Gq g4;  // + M L^T - B_lower P^T
+    g4.p(BD + i * bb, b, b, b, 1.0).t(M1, b, false, L, b, true, b, 1.0).t(BLp, b, false, P1, b, true, b, -1.0);
+    if (a > 0) {
+      g4.p(BRu + i * ab, a, b, a, 1.0).t(M1, b, false, R, b, true, b, 1.0).t(BLp, b, false, P2, b, true, b, -1.0);
+      g4.p(BRl + i * ab, b, a, b, 1.0).t(M2, b, false, L, b, true, b, 1.0).t(BRlp, b, false, P1, b, true, b, -1.0);
+      g4.p(BT, a, a, a, 1.0).t(M2, b, false, R, b, true, b, 1.0).t(BRlp, b, false, P2, b, true, b, -1.0);
+    }
+    if ((rc = g4.run(s))) return rc;
+    Gq g5;  // - P B_upper
+    g5.p(BD + i * bb, b, b, b, 1.0).t(P1, b, false, BUp_, b, false, b, -1.0);
+    if (a > 0) {
+      g5.p(BRu + i * ab, a, b, a, 1.0).t(P1, b, false, BRup, a, false, b, -1.0);
+      g5.p(BRl + i * ab, b, a, b, 1.0).t(P2, b, false, BUp_, b, false, b, -1.0);
+      g5.p(BT, a, a, a, 1.0).t(P2, b, false, BRup, a, false, b, -1.0);
+    }
+    if ((rc = g5.run(s))) return rc;
+  }
+  if (!invert_last_block) return ROCBTA_OK;
+  double* Dl = D + (n - 1) * bb;  // ddbtasc.py:447-475
+  double* Bl = BD + (n - 1) * bb;
+  rc = block_inverse(Dl, b, (int)b, ws, lay, s);
+  if (rc) return rc;
+  Gq h1;
+  h1.p(Z, b, b, b, 0.0).t(Dl, b, false, Bl, b, false, b, 1.0);
+  if (a > 0) h1.p(V, a, b, a, 0.0).t(Dl, b, false, Ru + (n - 1) * ab, a, false, b, 1.0);
+  if ((rc = h1.run(s))) return rc;
+  Gq h2;
+  h2.p(Bl, b, b, b, 0.0).t(Z, b, false, Dl, b, true, b, 1.0);
+  if (a > 0) h2.p(T, a, a, a, 1.0).t(Rl + (n - 1) * ab, b, false, V, a, false, b, -1.0);
+  if ((rc = h2.run(s))) return rc;
+  if (a == 0) return ROCBTA_OK;
+  rc = block_inverse(T, a, (int)a, ws, lay, s);
+  if (rc) return rc;
+  const double* R = Rl + (n - 1) * ab;
+  Gq h3;
+  h3.p(P2, b, a, b, 0.0).t(R, b, false, Dl, b, false, b, 1.0);
+  h3.p(M2, b, a, b, 0.0).t(R, b, false, Bl, b, false, b, 1.0);
+  if ((rc = h3.run(s))) return rc;
+  Gq h4;  // BT + R B' R^T - B_lower_arrow (R D)^T ...
+  h4.p(BT, a, a, a, 1.0).t(M2, b, false, R, b, true, b, 1.0).t(BRl + (n - 1) * ab, b, false, P2, b, true, b, -1.0);
+  if ((rc = h4.run(s))) return rc;
+  Gq h5;  // ... - (R D) B_upper_arrow
+  h5.p(BT, a, a, a, 1.0).t(P2, b, false, BRu + (n - 1) * ab, a, false, b, -1.0);
+  if ((rc = h5.run(s))) return rc;
+  Gq h6;  // T (..) T^T
+  h6.p(Y1, a, a, a, 0.0).t(T, a, false, BT, a, false, a, 1.0);
+  if ((rc = h6.run(s))) return rc;
+  Gq h7;
+  h7.p(BT, a, a, a, 0.0).t(Y1, a, false, T, a, true, a, 1.0);
+  return h7.run(s);
+}
+
+int rocbta_dddbtasci_quadratic(int64_t n, int64_t b, int64_t a, double* D, double* Lo,
+                               double* Up, double* Rl, double* Ru, double* T, double* BD,
+                               double* BLo, double* BUp, double* BRl, double* BRu, double* BT,
+                               int invert_last_block, void* workspace, size_t workspace_bytes,
+                               void* stream) {
+  if (bad_shape(n, b, a) || !D || !BD || (n > 1 && (!Lo || !Up || !BLo || !BUp)) ||
+      (a > 0 && (!Rl || !Ru || !T || !BRl || !BRu || !BT)))
+    return ROCBTA_ERR_ARG;
+  const DdLayout lay = dd_layout(b, a);
+  if (!workspace || workspace_bytes < lay.total) return ROCBTA_ERR_WORKSPACE;
+  hipStream_t s = (hipStream_t)stream;
+  char* ws = (char*)workspace;
+  auto P = [&](size_t off) { return (double*)(ws + off); };
+  double *B1 = P(lay.b1), *B2 = P(lay.b2), *C1 = P(lay.c1), *C2 = P(lay.c2), *E = P(lay.e);
+  double *Gm = P(lay.g), *Dc = P(lay.dc), *Bdc = P(lay.bdc), *DF1 = P(lay.df1), *DF2 = P(lay.df2);
+  double *DK1 = P(lay.dk1), *DK2 = P(lay.dk2), *S = P(lay.sm), *Mt = P(lay.mt), *Q1 = P(lay.q1);
+  double *Q2 = P(lay.q2), *Q3 = P(lay.q3), *Y = P(lay.y), *V1 = P(lay.v1), *Zt = P(lay.zt);
+  double *P3 = P(lay.p3), *D2 = P(lay.d2), *BB = P(lay.bbt), *Y2 = P(lay.y2);
+  const int64_t bb = b * b, ab = a * b;
+  int rc = ROCBTA_OK;
+  if (invert_last_block && a > 0) {  // ddbtasci.py:377-430
+    double* Ad = D + (n - 1) * bb;
+    double* Rul = Ru + (n - 1) * ab;
+    double* Rll = Rl + (n - 1) * ab;
+    double* Bd = BD + (n - 1) * bb;
+    double* t13 = BRu + (n - 1) * ab;
+    double* t31 = BRl + (n - 1) * ab;
+    Gq m1;
+    m1.p(B2, a, b, a, 0.0).t(Ad, b, false, Rul, a, false, b, 1.0);
+    m1.p(Y, b, a, b, 0.0).t(Rll, b, false, Bd, b, false, b, 1.0);
+    m1.p(V1, a, b, a, 0.0).t(Bd, b, false, Rll, b, true, b, 1.0);
+    m1.p(Zt, a, b, a, 0.0).t(Ad, b, false, t13, a, false, b, 1.0);
+    m1.p(P3, b, a, b, 0.0).t(T, a, false, t31, b, false, a, 1.0);
+    m1.p(Y2, b, a, b, 0.0).t(Rll, b, false, Ad, b, false, b, 1.0);
+    if ((rc = m1.run(s))) return rc;
+    Gq m2;  // D2 = T Rl B, BB = B2 T
+    m2.p(D2, b, a, b, 0.0).t(T, a, false, Y, b, false, a, 1.0);
+    m2.p(BB, a, b, a, 0.0).t(B2, a, false, T, a, false, a, 1.0);
+    if ((rc = m2.run(s))) return rc;
+    Gq m3;
+    m3.p(t13, a, b, a, 0.0).t(B2, a, false, BT, a, false, a, -1.0).t(V1, a, false, T, a, true, a, -1.0);
+    m3.p(t31, b, a, b, 0.0).t(BT, a, false, B2, a, true, a, -1.0).t(P3, b, false, Ad, b, true, b, 1.0);
+    if ((rc = m3.run(s))) return rc;
+    Gq m4;
+    m4.p(t13, a, b, a, 1.0).t(Zt, a, false, T, a, true, a, 1.0);
+    m4.p(t31, b, a, b, 1.0).t(T, a, false, Y, b, false, a, -1.0);
+    if ((rc = m4.run(s))) return rc;
+    // B_D = B_D - B2 X^B_{t,n-1} + D2^T B2^T - Zt (B2 T)^T
+    Gq m5;
+    m5.p(Bd, b, b, b, 1.0).t(B2, a, false, t31, b, false, a, -1.0).t(D2, b, true, B2, a, true, a, 1.0);
+    if ((rc = m5.run(s))) return rc;
+    Gq m6;
+    m6.p(Bd, b, b, b, 1.0).t(Zt, a, false, BB, a, true, a, -1.0);
+    // A: X_{t,n-1} = -T (Rl Ad), X_{n-1,t} = -B2 T
+    m6.p(Rll, b, a, b, 0.0).t(T, a, false, Y2, b, false, a, -1.0);
+    m6.p(Rul, a, b, a, 0.0).t(B2, a, false, T, a, false, a, -1.0);
+    if ((rc = m6.run(s))) return rc;
+    Gq m7;
+    m7.p(Ad, b, b, b, 1.0).t(B2, a, false, Rll, b, false, a, -1.0);
+    if ((rc = m7.run(s))) return rc;
+  }
+  for (int64_t i = n - 2; i >= 0; --i) {  // ddbtasci.py:432-576
+    double* Di = D + i * bb;
+    const double* D1 = D + (i + 1) * bb;
+    double* Li = Lo + i * bb;
+    double* Ui = Up + i * bb;
+    double* Rli = Rl + i * ab;
+    double* Rui = Ru + i * ab;
+    const double* Rl1 = Rl + (i + 1) * ab;
+    const double* Ru1 = Ru + (i + 1) * ab;
+    double* Bi = BD + i * bb;
+    const double* Bd1 = BD + (i + 1) * bb;
+    const double* Bl1 = BRl + (i + 1) * ab;
+    const double* Bu1 = BRu + (i + 1) * ab;
+    double* b12 = BUp + i * bb;
+    double* b13 = BRu + i * ab;
+    double* b21 = BLo + i * bb;
+    double* b31 = BRl + i * ab;
+    RB_CHECK(hipMemcpyAsync(Dc, Di, bb * sizeof(double), hipMemcpyDeviceToDevice, s));
+    RB_CHECK(hipMemcpyAsync(Bdc, Bi, bb * sizeof(double), hipMemcpyDeviceToDevice, s));
+    Gq la;
+    la.p(B1, b, b, b, 0.0).t(Ui, b, false, D1, b, false, b, 1.0).t(Rui, a, false, Rl1, b, false, a, 1.0);
+    la.p(C1, b, b, b, 0.0).t(D1, b, false, Li, b, false, b, 1.0).t(Ru1, a, false, Rli, b, false, a, 1.0);
+    la.p(DF1, b, b, b, 0.0).t(Ui, b, false, Bd1, b, false, b, 1.0).t(Rui, a, false, Bl1, b, false, a, 1.0);
+    la.p(DK1, b, b, b, 0.0).t(D1, b, false, b21, b, false, b, 1.0).t(Ru1, a, false, b31, b, false, a, 1.0);
+    if (a > 0) {
+      la.p(B2, a, b, a, 0.0).t(Ui, b, false, Ru1, a, false, b, 1.0).t(Rui, a, false, T, a, false, a, 1.0);
+      la.p(C2, b, a, b, 0.0).t(Rl1, b, false, Li, b, false, b, 1.0).t(T, a, false, Rli, b, false, a, 1.0);
+      la.p(DF2, a, b, a, 0.0).t(Ui, b, false, Bu1, a, false, b, 1.0).t(Rui, a, false, BT, a, false, a, 1.0);
+      la.p(DK2, b, a, b, 0.0).t(Rl1, b, false, b21, b, false, b, 1.0).t(T, a, false, b31, b, false, a, 1.0);
+    }
+    if ((rc = la.run(s))) return rc;
+    Gq lc;  // DF -= G, DK -= H
+    lc.p(DF1, b, b, b, 1.0).t(b12, b, false, D1, b, true, b, -1.0).t(b13, a, false, Ru1, a, true, a, -1.0);
+    lc.p(DK1, b, b, b, 1.0).t(Bd1, b, false, Ui, b, true, b, -1.0).t(Bu1, a, false, Rui, a, true, a, -1.0);
+    if (a > 0) {
+      lc.p(DF2, a, b, a, 1.0).t(b12, b, false, Rl1, b, true, b, -1.0).t(b13, a, false, T, a, true, a, -1.0);
+      lc.p(DK2, b, a, b, 1.0).t(Bl1, b, false, Ui, b, true, b, -1.0).t(BT, a, false, Rui, a, true, a, -1.0);
+    }
+    if ((rc = lc.run(s))) return rc;
+    Gq ld;
+    ld.p(E, b, b, b, 0.0).t(B1, b, false, Li, b, false, b, 1.0).t(B2, a, false, Rli, b, false, a, 1.0);
+    ld.p(S, b, b, b, 0.0).t(DF1, b, false, Ui, b, true, b, 1.0).t(DF2, a, false, Rui, a, true, a, 1.0);
+    ld.p(Mt, b, b, b, 0.0).t(Ui, b, false, C1, b, false, b, 1.0).t(Rui, a, false, C2, b, false, a, 1.0);
+    if ((rc = ld.run(s))) return rc;
+    Gq le;
+    le.p(S, b, b, b, 1.0).t(B1, b, false, b21, b, false, b, -1.0).t(B2, a, false, b31, b, false, a, -1.0);
+    if ((rc = le.run(s))) return rc;
+    Gq lf;
+    lf.p(Q1, b, b, b, 0.0).t(Dc, b, false, S, b, false, b, 1.0);
+    lf.p(Q2, b, b, b, 0.0).t(Dc, b, false, E, b, false, b, 1.0);
+    lf.p(Q3, b, b, b, 0.0).t(Bdc, b, true, Mt, b, true, b, 1.0);
+    if ((rc = lf.run(s))) return rc;
+    Gq lg;
+    lg.p(Bi, b, b, b, 1.0).t(Q1, b, false, Dc, b, true, b, 1.0).t(Q2, b, false, Bdc, b, false, b, 1.0);
+    lg.p(b12, b, b, b, 0.0).t(Dc, b, false, DF1, b, false, b, -1.0).t(Bdc, b, false, C1, b, true, b, -1.0);
+    lg.p(b21, b, b, b, 0.0).t(DK1, b, false, Dc, b, true, b, 1.0).t(C1, b, false, Bdc, b, false, b, -1.0);
+    if (a > 0) {
+      lg.p(b13, a, b, a, 0.0).t(Dc, b, false, DF2, a, false, b, -1.0).t(Bdc, b, false, C2, b, true, b, -1.0);
+      lg.p(b31, b, a, b, 0.0).t(DK2, b, false, Dc, b, true, b, 1.0).t(C2, b, false, Bdc, b, false, b, -1.0);
+    }
+    if ((rc = lg.run(s))) return rc;
+    Gq lh;
+    lh.p(Bi, b, b, b, 1.0).t(Q3, b, false, Dc, b, true, b, 1.0);
+    // A's selected inverse (ddbtasci.py:562-576)
+    lh.p(Ui, b, b, b, 0.0).t(Dc, b, false, B1, b, false, b, -1.0);
+    if (a > 0) lh.p(Rui, a, b, a, 0.0).t(Dc, b, false, B2, a, false, b, -1.0);
+    if ((rc = lh.run(s))) return rc;
+    Gq li;
+    li.p(Li, b, b, b, 0.0).t(C1, b, false, Dc, b, false, b, -1.0);
+    if (a > 0) li.p(Rli, b, a, b, 0.0).t(C2, b, false, Dc, b, false, b, -1.0);
+    li.p(Gm, b, b, b, 0.0).t(Dc, b, false, E, b, false, b, 1.0);
+    if ((rc = li.run(s))) return rc;
+    Gq lj;
+    lj.p(Di, b, b, b, 1.0).t(Gm, b, false, Dc, b, false, b, 1.0);
+    if ((rc = lj.run(s))) return rc;
   }
   return ROCBTA_OK;
 }

@@ -144,3 +144,15 @@ def test_ddbt_workspace_and_argument_errors(lib):
                                None) == _lib.ROCBTA_ERR_ARG
     assert lib.rocbta_dddbtasci(3, 4, 2, fake, fake, fake, fake, fake, fake, 1, 0, fake, 8,
                                 None) == _lib.ROCBTA_ERR_WORKSPACE
+
+
+def test_ddbt_quadratic_argument_errors(lib):
+    fake = ctypes.c_void_p(16)
+    big = 1 << 26
+    # missing B arrays, then a workspace too small
+    assert lib.rocbta_dddbtasc_quadratic(3, 4, 2, fake, fake, fake, fake, fake, fake, None, fake,
+                                         fake, fake, fake, fake, 1, fake, big, None) == _lib.ROCBTA_ERR_ARG
+    assert lib.rocbta_dddbtasci_quadratic(3, 4, 2, fake, fake, fake, fake, fake, fake, fake, fake,
+                                          fake, fake, None, fake, 1, fake, big, None) == _lib.ROCBTA_ERR_ARG
+    assert lib.rocbta_dddbtasci_quadratic(3, 4, 2, *([fake] * 12), 1, fake, 8,
+                                          None) == _lib.ROCBTA_ERR_WORKSPACE

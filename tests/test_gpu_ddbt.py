@@ -10,7 +10,9 @@ Gauss-Jordan without pivoting, the oracle by numpy.linalg.inv (LU with
 partial pivoting); on these matrices both are accurate to ~1e-15.
 Shapes: the reference grid (b, a in {2, 3}, n 1..4), 64-wide tile
 boundaries of the blocked inverse (b 63, 64, 65, 100, 130, 256; a 1, 16,
-64, 65), the arrowless forms in both directions.
+64, 65), the arrowless forms in both directions; the A X A^T = B forms
+(symmetric B, as the reference's tests) against the oracle and against
+A^{-1} B A^{-T} from dense linear algebra.
 """
 
 import numpy as np
@@ -100,11 +102,59 @@ def test_ddbt_argument_errors(algs, gpu_device):
     g = to_dev(gpu_device, *dense.bta_dense_to_arrays_full(A, b, a, n))
     with pytest.raises(TypeError, match="Unexpected keyword"):
         algs.ddbtasc(*g, direction="upward")
-    with pytest.raises(NotImplementedError, match="rhs"):
+    with pytest.raises(ValueError, match="rhs does not contain"):
         algs.ddbtasc(*g, rhs={"B_diagonal_blocks": g[0]}, quadratic=True)
+    rhs = dict(zip(("B_diagonal_blocks", "B_lower_diagonal_blocks", "B_upper_diagonal_blocks",
+                    "B_lower_arrow_blocks", "B_upper_arrow_blocks", "B_arrow_tip_block"),
+                   [x.clone() for x in g]))
+    with pytest.raises(NotImplementedError, match="AX=B"):
+        algs.ddbtasc(*g, rhs=rhs)
+    bt = {k: rhs[k] for k in list(rhs)[:3]}
+    with pytest.raises(NotImplementedError, match="upward"):
+        algs.ddbtsc(g[0], g[1], g[2], rhs=bt, quadratic=True, direction="upward")
     with pytest.raises(NotImplementedError, match="buffers"):
         algs.ddbtasci(*g, buffers={})
     with pytest.raises(ValueError, match="direction"):
         algs.ddbtsc(g[0], g[1], g[2], direction="sideways")
     with pytest.raises(ValueError, match="upper arrow"):
         algs.ddbtasc(g[0], g[1], g[2], g[3], g[4][:, :, :1].contiguous(), g[5])
+
+
+QSHAPES = [(2, 2, 1), (3, 3, 4), (64, 16, 4), (65, 64, 3), (100, 3, 5), (130, 65, 2), (64, 0, 4)]
+
+
+@pytest.mark.parametrize("b,a,n", QSHAPES)
+def test_ddbt_quadratic_matches_oracle_and_dense(algs, gpu_device, b, a, n):
+    print(f"ddbt quadratic b={b} a={a} n={n}", flush=True)
+    rng = np.random.default_rng(44000 + b + 7 * a + 13 * n)
+    A = generators.dd_bta(b, a, n, rng)
+    Bm = generators.dd_bta(b, a, n, rng)
+    generators.symmetrize(Bm)
+    arrs = dense.bta_dense_to_arrays_full(A, b, a, n)
+    brr = dense.bta_dense_to_arrays_full(Bm, b, a, n)
+    g, gb = to_dev(gpu_device, *arrs), to_dev(gpu_device, *brr)
+    keys = ("B_diagonal_blocks", "B_lower_diagonal_blocks", "B_upper_diagonal_blocks",
+            "B_lower_arrow_blocks", "B_upper_arrow_blocks", "B_arrow_tip_block")
+    if a:
+        rhs = dict(zip(keys, gb))
+        algs.ddbtasc(*g, rhs=rhs, quadratic=True)
+    else:
+        rhs = dict(zip(keys[:3], gb[:3]))
+        algs.ddbtsc(g[0], g[1], g[2], rhs=rhs, quadratic=True)
+    o, ob = [x.copy() for x in arrs], [x.copy() for x in brr]
+    serinv_np.ddbtasc_quadratic(*o, *ob)
+    for x, y in zip(to_host(*g, *gb), o + ob):
+        assert relerr(x, y) < TOL
+    if a:
+        algs.ddbtasci(*g, rhs=rhs, quadratic=True)
+    else:
+        algs.ddbtsci(g[0], g[1], g[2], rhs=rhs, quadratic=True)
+    serinv_np.ddbtasci_quadratic(*o, *ob)
+    h = to_host(*g, *gb)
+    for x, y in zip(h, o + ob):
+        assert relerr(x, y) < TOL
+    X = np.linalg.inv(A)
+    ref = list(dense.bta_dense_to_arrays_full(X, b, a, n)) + list(
+        dense.bta_dense_to_arrays_full(X @ Bm @ X.T, b, a, n))
+    for x, y in zip(h, ref):
+        assert relerr(x, y) < TOL

@@ -124,3 +124,29 @@ def test_ddbt_workload_n8_b32_a8():
     serinv_np.ddbtasci(*arrs)
     for x, y in zip(arrs, dense.bta_dense_to_arrays_full(X, b, a, n)):
         assert relerr(x, y) < 1e-11
+
+
+@pytest.mark.parametrize("b,a,n", [(b, a, n) for b in (2, 3) for a in (0, 2, 3) for n in (1, 2, 3, 4)])
+def test_ddbtasc_ddbtasci_quadratic_vs_dense(b, a, n):
+    """A X A^T = B (test_ddbtasci.py:37-60, 108-121): after both sweeps the
+    A arrays hold A^{-1} and the B arrays A^{-1} B A^{-T} on the pattern.
+    B is symmetric, as the reference's tests make it: its recurrences use
+    B_ii^T in places of B_ii (ddbtasci.py:540-544)."""
+    rng = np.random.default_rng(36000 + 100 * b + 10 * a + n)
+    A = generators.dd_bta(b, a, n, rng)
+    B = generators.dd_bta(b, a, n, rng)
+    generators.symmetrize(B)
+    X = np.linalg.inv(A)
+    XBX = X @ B @ X.T
+    arrs = list(dense.bta_dense_to_arrays_full(A, b, a, n))
+    brr = list(dense.bta_dense_to_arrays_full(B, b, a, n))
+    serinv_np.ddbtasc_quadratic(*arrs, brr[0], brr[1], brr[2], brr[3], brr[4], brr[5])
+    # the tip after the sweep (test_ddbtasc.py:108-121)
+    if a:
+        assert relerr(brr[5], XBX[n * b:, n * b:]) < RTOL
+    serinv_np.ddbtasci_quadratic(*arrs, brr[0], brr[1], brr[2], brr[3], brr[4], brr[5])
+    for x, y in zip(arrs, dense.bta_dense_to_arrays_full(X, b, a, n)):
+        assert relerr(x, y) < RTOL
+    for x, y in zip(brr, dense.bta_dense_to_arrays_full(XBX, b, a, n)):
+        assert relerr(x, y) < RTOL
+
