@@ -45,6 +45,7 @@ def parse():
     p.add_argument("--no-mixed", action="store_true")
     p.add_argument("--no-permuted", action="store_true")
     p.add_argument("--no-streaming", action="store_true")
+    p.add_argument("--no-ddbt", action="store_true")
     return p.parse_args()
 
 
@@ -374,6 +375,48 @@ def bench_streaming(dev, n, b, a, reps=2):
     return res
 
 
+def bench_ddbt(dev, n, b, a, reps=2):
+    """Non-SPD family: ddbtasc + ddbtasci on a diagonally dominant
+    non-symmetric BTA matrix (random blocks in [0, 1), every diagonal entry
+    raised by 3b + a + 1, generated on the device).  Property check: the
+    diagonal blocks of A X are the identity, (A X)_ii = A_{i,i-1} X_{i-1,i}
+    + A_ii X_ii + A_{i,i+1} X_{i+1,i} + A_{i,t} X_{t,i}."""
+    import torch
+
+    import serinv_amd.algs as algs
+
+    g = torch.Generator(device=dev)
+    g.manual_seed(63)
+    rnd = lambda *shape: torch.rand(*shape, generator=g, dtype=torch.float64, device=dev)  # noqa: E731
+    D0 = rnd(n, b, b) + (3 * b + a + 1) * torch.eye(b, dtype=torch.float64, device=dev)
+    Lo0, Up0, Rl0, Ru0 = rnd(n - 1, b, b), rnd(n - 1, b, b), rnd(n, a, b), rnd(n, b, a)
+    T0 = rnd(a, a) + (n * b + a + 1) * torch.eye(a, dtype=torch.float64, device=dev)
+    stream = torch.cuda.current_stream(dev)
+    t_sc, t_sci = [], []
+    for _ in range(reps):
+        D, Lo, Up, Rl, Ru, T = (x.clone() for x in (D0, Lo0, Up0, Rl0, Ru0, T0))
+        e = [torch.cuda.Event(enable_timing=True) for _ in range(3)]
+        e[0].record(stream)
+        algs.ddbtasc(D, Lo, Up, Rl, Ru, T)
+        e[1].record(stream)
+        algs.ddbtasci(D, Lo, Up, Rl, Ru, T)
+        e[2].record(stream)
+        torch.cuda.synchronize()
+        t_sc.append(e[0].elapsed_time(e[1]))
+        t_sci.append(e[1].elapsed_time(e[2]))
+    eye = torch.eye(b, dtype=torch.float64, device=dev)
+    err = 0.0
+    for i in range(n):
+        ax = D0[i] @ D[i] + Ru0[i] @ Rl[i]
+        if i > 0:
+            ax = ax + Lo0[i - 1] @ Up[i - 1]
+        if i + 1 < n:
+            ax = ax + Up0[i] @ Lo[i]
+        err = max(err, float((ax - eye).abs().max()))
+    return {"n": n, "b": b, "a": a, "ddbtasc_ms": round(min(t_sc), 3),
+            "ddbtasci_ms": round(min(t_sci), 3), "max_abs_AX_ii_minus_I": float(f"{err:.3e}")}
+
+
 def bench_mixed(dev, n, b, a, kappas=(1e4, 1e6, 1e8), r=16, reps=2):
     """Mixed FP32/FP64 against plain FP64 on the calibrated matrices of the
     given shape (cond ~ kappa): pobtaf + pobtasi time of both forms, max
@@ -627,6 +670,10 @@ def main():
         print("[bench] host-streaming form, n=64 ...", file=sys.stderr, flush=True)
         out["streaming"] = bench_streaming(dev, 64, b, a)
         print(f"[bench] streaming {out['streaming']}", file=sys.stderr, flush=True)
+    if not args.no_ddbt:
+        print("[bench] non-SPD ddbtasc + ddbtasci, n=64 ...", file=sys.stderr, flush=True)
+        out["ddbt"] = bench_ddbt(dev, 64, b, a)
+        print(f"[bench] ddbt {out['ddbt']}", file=sys.stderr, flush=True)
     if not args.no_batched:
         print("[bench] batched plan vs per-matrix loop ...", file=sys.stderr, flush=True)
         out["batched"] = bench_batched(dev)
