@@ -282,20 +282,40 @@ int rocbta_dddbtasci(int64_t n, int64_t b, int64_t a, double* D, double* Lo, dou
 /* The A X A^T = B forms ("quadratic"): the same sweeps with the blocks of a
  * symmetric B (BD, BLo, BUp, BRl, BRu, BT, same shapes as A's) carried
  * along; after both sweeps A's arrays hold A^{-1} and B's A^{-1} B A^{-T}
- * on the pattern.  Downward only (a == 0 gives the BT forms).  Replace
+ * on the pattern.  direction 0 = downward; 1 = upward, BT only (a == 0),
+ * and the selected inversion must use the sweep's direction.  Replace
  * _ddbtasc_quadratic (ddbtasc.py:335-475) / _ddbtsc_quadratic (ddbtsc.py:
- * 271-311) and _ddbtasci_quadratic (ddbtasci.py:343-576) / _ddbtsci_quadratic
- * (ddbtsci.py:299-369).  Workspace: ROCBTA_OP_DDBTASC. */
+ * 271-311) / _ddbtsc_upward_quadratic (ddbtsc.py:314-356) and
+ * _ddbtasci_quadratic (ddbtasci.py:343-576) / _ddbtsci_quadratic
+ * (ddbtsci.py:299-369) / _ddbtsci_upward_quadratic (ddbtsci.py:372-445).
+ * Workspace: ROCBTA_OP_DDBTASC. */
 int rocbta_dddbtasc_quadratic(int64_t n, int64_t b, int64_t a, double* D, double* Lo, double* Up,
                               double* Rl, double* Ru, double* T, double* BD, double* BLo,
                               double* BUp, double* BRl, double* BRu, double* BT,
-                              int invert_last_block, void* workspace, size_t workspace_bytes,
-                              void* stream);
+                              int invert_last_block, int direction, void* workspace,
+                              size_t workspace_bytes, void* stream);
 int rocbta_dddbtasci_quadratic(int64_t n, int64_t b, int64_t a, double* D, double* Lo,
                                double* Up, double* Rl, double* Ru, double* T, double* BD,
                                double* BLo, double* BUp, double* BRl, double* BRu, double* BT,
-                               int invert_last_block, void* workspace, size_t workspace_bytes,
-                               void* stream);
+                               int invert_last_block, int direction, void* workspace,
+                               size_t workspace_bytes, void* stream);
+
+/* Permuted ("middle partition") forms: the sweep eliminates the middle
+ * blocks 1 .. n-2 top-down and leaves blocks 0, n-1 and the tip as the
+ * reduced system; Lbuf and Ubuf ((n-1) x b x b, like Lo) carry the fill-ins
+ * A~_{0,k+1} / A~_{k+1,0} of row and column 0.  The selected inversion
+ * expects the reduced system's inverse in blocks 0, n-1, the tip, their
+ * arrow blocks, Lbuf[n-2] and Ubuf[n-2]; on exit Lbuf[k] = X_{0,k+1},
+ * Ubuf[k] = X_{k+1,0}.  n >= 2.  Replace _ddbtasc_permuted (ddbtasc.py:
+ * 244-332) / _ddbtsc_permuted (ddbtsc.py:224-269) and _ddbtasci_permuted
+ * (ddbtasci.py:259-341) / _ddbtsci_permuted (ddbtsci.py:240-297).
+ * Workspace: ROCBTA_OP_DDBTASC. */
+int rocbta_dddbtasc_permuted(int64_t n, int64_t b, int64_t a, double* D, double* Lo, double* Up,
+                             double* Rl, double* Ru, double* T, double* Lbuf, double* Ubuf,
+                             void* workspace, size_t workspace_bytes, void* stream);
+int rocbta_dddbtasci_permuted(int64_t n, int64_t b, int64_t a, double* D, double* Lo, double* Up,
+                              double* Rl, double* Ru, double* T, double* Lbuf, double* Ubuf,
+                              void* workspace, size_t workspace_bytes, void* stream);
 
 /* -------------------------------------------------------------- streaming
  * Host <-> device streaming forms for systems whose blocks do not fit in

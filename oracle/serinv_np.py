@@ -383,6 +383,71 @@ def ddbtsci(D, Lo, Up, direction="downward"):
         raise ValueError("Optional keyword argument `direction` must be either 'downward' or 'upward'")
 
 
+def ddbtsc_quadratic(D, Lo, Up, BD, BLo, BUp, invert_last_block=True, direction="downward"):
+    """A X A^T = B sweep of a BT matrix: downward src/serinv/algs/ddbtsc.py:271-311,
+    upward :314-356.  Each step inverts the block being eliminated, maps its
+    B block to A_ii^{-1} B_ii A_ii^{-T}, and updates the neighbour's A and B
+    blocks (loops ddbtsc.py:285-305 downward, :328-348 upward)."""
+    n = D.shape[0]
+    if direction == "downward":
+        for i in range(0, n - 1):
+            D[i] = np.linalg.inv(D[i])
+            BD[i] = D[i] @ BD[i] @ D[i].T
+            LD = Lo[i] @ D[i]  # A_{i+1,i} A_ii^{-1}
+            D[i + 1] = D[i + 1] - LD @ Up[i]
+            BD[i + 1] = BD[i + 1] + Lo[i] @ BD[i] @ Lo[i].T - BLo[i] @ LD.T - LD @ BUp[i]
+        last = n - 1
+    elif direction == "upward":
+        for i in range(n - 1, 0, -1):
+            D[i] = np.linalg.inv(D[i])
+            BD[i] = D[i] @ BD[i] @ D[i].T
+            UD = Up[i - 1] @ D[i]  # A_{i-1,i} A_ii^{-1}
+            D[i - 1] = D[i - 1] - UD @ Lo[i - 1]
+            BD[i - 1] = BD[i - 1] + Up[i - 1] @ BD[i] @ Up[i - 1].T - BUp[i - 1] @ UD.T - UD @ BLo[i - 1]
+        last = 0
+    else:
+        raise ValueError("Optional keyword argument `direction` must be either 'downward' or 'upward'")
+    if invert_last_block:  # :307-311, :350-356
+        D[last] = np.linalg.inv(D[last])
+        BD[last] = D[last] @ BD[last] @ D[last].T
+
+
+def ddbtsci_quadratic(D, Lo, Up, BD, BLo, BUp, direction="downward"):
+    """Selected inversion for A X A^T = B of a BT matrix from ddbtsc_quadratic's
+    output (same direction): downward src/serinv/algs/ddbtsci.py:299-369
+    (loop :320), upward :372-445 (loop :393).  Each step pairs the finished
+    block f (X_ff, XB_ff) with its neighbour g, which holds the Schur inverse
+    S_g^{-1} and S_g^{-1} B~_g S_g^{-T}.  In g/f roles both directions run
+    the same formulas: P = S_g^{-1} A_gf (the reference's temp_1 downward,
+    temp_3 upward) and Q = X_ff A_fg (temp_3 downward, temp_1 upward).  For a
+    symmetric B (the reference uses B_gg^T for B_gg)."""
+    n = D.shape[0]
+    if direction == "downward":
+        # g = i, f = i + 1; A_gf = Up[i], A_fg = Lo[i]
+        steps = [(i, i + 1, i) for i in range(n - 2, -1, -1)]
+        AU, AL, BU, BL = Up, Lo, BUp, BLo
+    elif direction == "upward":
+        # g = i, f = i - 1; A_gf = Lo[i-1], A_fg = Up[i-1]
+        steps = [(i, i - 1, i - 1) for i in range(1, n)]
+        AU, AL, BU, BL = Lo, Up, BLo, BUp
+    else:
+        raise ValueError("Optional keyword argument `direction` must be either 'downward' or 'upward'")
+    for g, f, k in steps:
+        Bgf, Bfg = BU[k].copy(), BL[k].copy()
+        t1 = D[g] @ AU[k]         # P
+        t4 = AL[k].T @ D[f].T     # Q^T
+        t2 = AU[k].T @ D[g].T     # P^T
+        t3 = D[f] @ AL[k]         # Q
+        BU[k] = -t1 @ BD[f] - BD[g] @ t4 + D[g] @ Bgf @ D[f].T
+        BL[k] = -BD[f] @ t2 - t3 @ BD[g] + D[f] @ Bfg @ D[g].T
+        BD[g] = (BD[g] + t1 @ BD[f] @ t2 + t1 @ t3 @ BD[g] + BD[g].T @ t4 @ t2
+                 - t1 @ D[f] @ Bfg @ D[g].T - D[g] @ Bgf @ D[f].T @ t2)
+        Afg = AL[k].copy()
+        AL[k] = -t3 @ D[g]
+        AU[k] = -t1 @ D[f]
+        D[g] = D[g] - AU[k] @ Afg @ D[g]
+
+
 def ddbtasc_quadratic(D, Lo, Up, Rl, Ru, T, BD, BLo, BUp, BRl, BRu, BT, invert_last_block=True):
     """Schur-complement sweep for A X A^T = B, src/serinv/algs/ddbtasc.py:335-475:
     the A sweep of ddbtasc, with B's blocks carried through the same
@@ -457,3 +522,57 @@ def ddbtasci_quadratic(D, Lo, Up, Rl, Ru, T, BD, BLo, BUp, BRl, BRu, BT, invert_
         Lo[i] = -C1 @ D[i]
         Rl[i] = -C2 @ D[i]
         D[i] = D[i] + D[i] @ (B1 @ D1 + B2 @ D2) @ D[i]  # :571-576
+
+
+# --------------------------------------------------------------------------
+# Non-SPD permuted ("middle partition") forms.  Lbuf, Ubuf have the shape of
+# the lower blocks, (n-1, b, b): during the sweep Lbuf[k] holds the fill-in
+# A~_{0,k+1} of row 0 and Ubuf[k] the fill-in A~_{k+1,0} of column 0; after
+# the selected inversion they hold X_{0,k+1} and X_{k+1,0}.  The middle
+# blocks 1 .. n-2 are eliminated; blocks 0, n-1 and the tip form the reduced
+# system.
+# --------------------------------------------------------------------------
+def ddbtasc_permuted(D, Lo, Up, Rl, Ru, T, Lbuf, Ubuf):
+    """src/serinv/algs/ddbtasc.py:244-332 (with a = 0, ddbtsc.py:224-269)."""
+    n = D.shape[0]
+    Lbuf[0] = Up[0]  # A_{0,1}
+    Ubuf[0] = Lo[0]  # A_{1,0}
+    for i in range(1, n - 1):
+        D[i] = np.linalg.inv(D[i])
+        S = D[i]
+        D[i + 1] = D[i + 1] - Lo[i] @ S @ Up[i]
+        Rl[i + 1] = Rl[i + 1] - Rl[i] @ S @ Up[i]
+        Ru[i + 1] = Ru[i + 1] - Lo[i] @ S @ Ru[i]
+        T[:, :] = T - Rl[i] @ S @ Ru[i]
+        Lbuf[i] = -Lbuf[i - 1] @ S @ Up[i]
+        Ubuf[i] = -Lo[i] @ S @ Ubuf[i - 1]
+        D[0] = D[0] - Lbuf[i - 1] @ S @ Ubuf[i - 1]
+        Rl[0] = Rl[0] - Rl[i] @ S @ Ubuf[i - 1]
+        Ru[0] = Ru[0] - Lbuf[i - 1] @ S @ Ru[i]
+
+
+def ddbtasci_permuted(D, Lo, Up, Rl, Ru, T, Lbuf, Ubuf):
+    """src/serinv/algs/ddbtasci.py:259-341 (with a = 0, ddbtsci.py:240-297).
+    Expects blocks 0, n-1, the tip, their arrow blocks and Lbuf[-1], Ubuf[-1]
+    to hold the reduced system's inverse."""
+    n = D.shape[0]
+    for i in range(n - 2, 0, -1):
+        # row i of A~ times the finished columns, and the finished rows times column i
+        B1 = Up[i] @ D[i + 1] + Ru[i] @ Rl[i + 1] + Ubuf[i - 1] @ Lbuf[i]
+        B2 = Up[i] @ Ubuf[i] + Ubuf[i - 1] @ D[0] + Ru[i] @ Rl[0]
+        B3 = Up[i] @ Ru[i + 1] + Ru[i] @ T + Ubuf[i - 1] @ Ru[0]
+        C1 = D[i + 1] @ Lo[i] + Ru[i + 1] @ Rl[i] + Ubuf[i] @ Lbuf[i - 1]
+        C2 = Lbuf[i] @ Lo[i] + D[0] @ Lbuf[i - 1] + Ru[0] @ Rl[i]
+        C3 = Rl[i + 1] @ Lo[i] + T @ Rl[i] + Rl[0] @ Lbuf[i - 1]
+        S = D[i]
+        E = B1 @ Lo[i] + B2 @ Lbuf[i - 1] + B3 @ Rl[i]
+        Up[i] = -S @ B1
+        Ubuf[i - 1] = -S @ B2
+        Ru[i] = -S @ B3
+        Lo[i] = -C1 @ S
+        Lbuf[i - 1] = -C2 @ S
+        Rl[i] = -C3 @ S
+        D[i] = S + S @ E @ S
+    if n > 1:
+        Lo[0] = Ubuf[0]  # X_{1,0}
+        Up[0] = Lbuf[0]  # X_{0,1}

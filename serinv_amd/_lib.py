@@ -62,9 +62,13 @@ SIGNATURES = {
     "rocbta_dddbtasci": (_c_int, [_c_i64, _c_i64, _c_i64, _c_p, _c_p, _c_p, _c_p, _c_p, _c_p, _c_int,
                                   _c_int, _c_p, _c_size, _c_p]),
     "rocbta_dddbtasc_quadratic": (_c_int, [_c_i64, _c_i64, _c_i64] + [_c_p] * 12 +
-                                  [_c_int, _c_p, _c_size, _c_p]),
+                                  [_c_int, _c_int, _c_p, _c_size, _c_p]),
     "rocbta_dddbtasci_quadratic": (_c_int, [_c_i64, _c_i64, _c_i64] + [_c_p] * 12 +
-                                   [_c_int, _c_p, _c_size, _c_p]),
+                                   [_c_int, _c_int, _c_p, _c_size, _c_p]),
+    "rocbta_dddbtasc_permuted": (_c_int, [_c_i64, _c_i64, _c_i64] + [_c_p] * 8 +
+                                 [_c_p, _c_size, _c_p]),
+    "rocbta_dddbtasci_permuted": (_c_int, [_c_i64, _c_i64, _c_i64] + [_c_p] * 8 +
+                                  [_c_p, _c_size, _c_p]),
     "rocbta_dlogdet": (_c_int, [_c_i64, _c_i64, _c_i64, _c_p, _c_p, _c_p, _c_p, _c_size, _c_p]),
     "rocbta_dag_selfcheck": (_c_i64, [_c_i64, _c_i64, _c_i64]),
     "rocbta_mpobtaf": (_c_int, [_c_i64, _c_i64, _c_i64, _c_p, _c_p, _c_p, _c_p, _c_int, _c_p,

@@ -8,10 +8,14 @@ upward).  They run as rocbta_dddbtasc / rocbta_dddbtasci: blocked
 Gauss-Jordan inverses of the diagonally dominant diagonal blocks (no
 pivoting) and grouped GEMMs.  rhs= with quadratic=True ("AXA^T=B",
 ddbtasc.py:335-475, ddbtasci.py:343-576; downward) runs as
-rocbta_dddbtasc_quadratic / rocbta_dddbtasci_quadratic.  rhs= without
-quadratic (the reference's "AX=B" branch is an empty placeholder,
-ddbtasc.py:183-185), the permuted buffers= forms and the upward quadratic
-BT forms raise NotImplementedError.
+rocbta_dddbtasc_quadratic / rocbta_dddbtasci_quadratic; the upward BT
+quadratic forms (ddbtsc.py:314-356, ddbtsci.py:372-445) run through the
+same entry points with direction=1.  rhs= without quadratic (the
+reference's "AX=B" branch is an empty placeholder, ddbtasc.py:183-185) and
+the quadratic permuted forms raise NotImplementedError.  buffers= without
+rhs (ddbtasc.py:244-332, ddbtasci.py:259-341, ddbtsc.py:224-269,
+ddbtsci.py:240-297) runs as rocbta_dddbtasc_permuted /
+rocbta_dddbtasci_permuted.
 """
 
 import torch
@@ -54,21 +58,31 @@ _B_KEYS = ("B_diagonal_blocks", "B_lower_diagonal_blocks", "B_upper_diagonal_blo
 
 
 def _rhs_arrays(kwargs, what, arrow):
-    """The B blocks of the A X A^T = B form, or None for A X = I
-    (ddbtasc.py:74-126 for the checks and their messages)."""
-    if kwargs.get("buffers", None) is not None:
-        raise NotImplementedError(f"{what}: the permuted (buffers=) form is not provided")
+    """The B blocks of the A X A^T = B form, or None for A X = I, and the
+    permutation buffers (Lbuf, Ubuf) or None (ddbtasc.py:74-126 and
+    ddbtasci.py:76-110 for the checks and their messages)."""
     rhs = kwargs.get("rhs", None)
-    if rhs is None:
-        return None
-    keys = _B_KEYS if arrow else _B_KEYS[:3]
-    arrs = [rhs.get(k, None) for k in keys]
-    if any(x is None for x in arrs):
-        raise ValueError("rhs does not contain the correct arrays")
-    if not kwargs.get("quadratic", False):
-        raise NotImplementedError(f"{what}: the AX=B form (rhs= without quadratic=True) is not "
-                                  "provided")
-    return arrs
+    buffers = kwargs.get("buffers", None)
+    arrs = None
+    if rhs is not None:
+        keys = _B_KEYS if arrow else _B_KEYS[:3]
+        arrs = [rhs.get(k, None) for k in keys]
+        if any(x is None for x in arrs):
+            raise ValueError("rhs does not contain the correct arrays")
+        if not kwargs.get("quadratic", False):
+            raise NotImplementedError(f"{what}: the AX=B form (rhs= without quadratic=True) is not "
+                                      "provided")
+    bufs = None
+    if buffers is not None:
+        bufs = [buffers.get(k, None) for k in ("A_lower_buffer_blocks", "A_upper_buffer_blocks")]
+        if any(x is None for x in bufs):
+            raise ValueError("buffers does not contain the correct arrays for A")
+        if arrs is not None:
+            if any(buffers.get(k, None) is None
+                   for k in ("B_lower_buffer_blocks", "B_upper_buffer_blocks")):
+                raise ValueError("buffers does not contain the correct arrays for B")
+            raise NotImplementedError(f"{what}: the permuted A X A^T = B form is not provided")
+    return arrs, bufs
 
 
 def _check_like(B, A, name):
@@ -88,19 +102,28 @@ def _direction(kwargs):
 
 
 def _run(fn_name, op, D, Lo, Up, Rl, Ru, T, n, b, a, dev, invert_last_block, direction, what,
-         B=None):
+         B=None, bufs=None):
     lib = _lib.load()
+    if bufs is not None:  # permuted: the middle blocks 1 .. n-2
+        if n < 2:
+            raise IndexError(f"{what}: the permuted form needs at least 2 diagonal blocks, got {n}")
+        for x, nm in zip(bufs, ("A_lower_buffer_blocks", "A_upper_buffer_blocks")):
+            _check_like(x, Lo, nm)
+        ws, nbytes = workspace(op, n, b, a, 0, dev)
+        rc = getattr(lib, fn_name + "_permuted")(
+            n, b, a, ptr(D), ptr(Lo), ptr(Up), ptr(Rl), ptr(Ru), ptr(T), ptr(bufs[0]),
+            ptr(bufs[1]), ws.data_ptr(), nbytes, stream_of(dev))
+        _lib.check(rc, what)
+        return
     ws, nbytes = workspace(op, n, b, a, 0, dev)
-    if B is not None:  # A X A^T = B: downward only
-        if direction != 0:
-            raise NotImplementedError(f"{what}: the upward A X A^T = B form is not provided")
+    if B is not None:  # A X A^T = B
         for x, y, nm in zip(B, (D, Lo, Up, Rl, Ru, T), _B_KEYS):
             if y is not None:
                 _check_like(x, y, nm)
         Bp = [ptr(x) for x in B] + [None] * (6 - len(B))
         rc = getattr(lib, fn_name + "_quadratic")(
             n, b, a, ptr(D), ptr(Lo), ptr(Up), ptr(Rl), ptr(Ru), ptr(T), *Bp,
-            int(invert_last_block), ws.data_ptr(), nbytes, stream_of(dev))
+            int(invert_last_block), int(direction), ws.data_ptr(), nbytes, stream_of(dev))
         _lib.check(rc, what)
         return
     rc = getattr(lib, fn_name)(n, b, a, ptr(D), ptr(Lo), ptr(Up), ptr(Rl), ptr(Ru), ptr(T),
@@ -117,12 +140,12 @@ def ddbtasc(A_diagonal_blocks, A_lower_diagonal_blocks, A_upper_diagonal_blocks,
     the arrow blocks and the tip their Schur-updated values (the tip
     inverted with invert_last_block=True, the default)."""
     check_kwargs(kwargs, {"rhs", "quadratic", "buffers", "invert_last_block"})
-    B = _rhs_arrays(kwargs, "ddbtasc", True)
+    B, bufs = _rhs_arrays(kwargs, "ddbtasc", True)
     arrs = (A_diagonal_blocks, A_lower_diagonal_blocks, A_upper_diagonal_blocks,
             A_lower_arrow_blocks, A_upper_arrow_blocks, A_arrow_tip_block)
     n, b, a, dev = _shapes(*arrs)
     _run("rocbta_dddbtasc", _lib.OP_DDBTASC, *arrs, n, b, a, dev,
-         bool(kwargs.get("invert_last_block", True)), 0, "ddbtasc", B)
+         bool(kwargs.get("invert_last_block", True)), 0, "ddbtasc", B, bufs)
 
 
 def ddbtasci(A_diagonal_blocks, A_lower_diagonal_blocks, A_upper_diagonal_blocks,
@@ -130,32 +153,32 @@ def ddbtasci(A_diagonal_blocks, A_lower_diagonal_blocks, A_upper_diagonal_blocks
     """Selected inversion from the ddbtasc output, in place: the blocks of
     A^{-1} on the BTA pattern."""
     check_kwargs(kwargs, {"rhs", "quadratic", "buffers", "invert_last_block"})
-    B = _rhs_arrays(kwargs, "ddbtasci", True)
+    B, bufs = _rhs_arrays(kwargs, "ddbtasci", True)
     arrs = (A_diagonal_blocks, A_lower_diagonal_blocks, A_upper_diagonal_blocks,
             A_lower_arrow_blocks, A_upper_arrow_blocks, A_arrow_tip_block)
     n, b, a, dev = _shapes(*arrs)
     _run("rocbta_dddbtasci", _lib.OP_DDBTASCI, *arrs, n, b, a, dev,
-         bool(kwargs.get("invert_last_block", True)), 0, "ddbtasci", B)
+         bool(kwargs.get("invert_last_block", True)), 0, "ddbtasci", B, bufs)
 
 
 def ddbtsc(A_diagonal_blocks, A_lower_diagonal_blocks, A_upper_diagonal_blocks, **kwargs):
     """Schur-complement sweep of a diagonally dominant BT matrix, in place;
     direction "downward" (default) or "upward"."""
     check_kwargs(kwargs, {"rhs", "quadratic", "buffers", "invert_last_block", "direction"})
-    B = _rhs_arrays(kwargs, "ddbtsc", False)
+    B, bufs = _rhs_arrays(kwargs, "ddbtsc", False)
     direction = _direction(kwargs)
     D, Lo, Up = A_diagonal_blocks, A_lower_diagonal_blocks, A_upper_diagonal_blocks
     n, b, a, dev = _shapes(D, Lo, Up, None, None, None)
     _run("rocbta_dddbtasc", _lib.OP_DDBTASC, D, Lo, Up, None, None, None, n, b, 0, dev,
-         bool(kwargs.get("invert_last_block", True)), direction, "ddbtsc", B)
+         bool(kwargs.get("invert_last_block", True)), direction, "ddbtsc", B, bufs)
 
 
 def ddbtsci(A_diagonal_blocks, A_lower_diagonal_blocks, A_upper_diagonal_blocks, **kwargs):
     """Selected inversion from the ddbtsc output (same direction), in place."""
     check_kwargs(kwargs, {"rhs", "quadratic", "buffers", "direction"})
-    B = _rhs_arrays(kwargs, "ddbtsci", False)
+    B, bufs = _rhs_arrays(kwargs, "ddbtsci", False)
     direction = _direction(kwargs)
     D, Lo, Up = A_diagonal_blocks, A_lower_diagonal_blocks, A_upper_diagonal_blocks
     n, b, a, dev = _shapes(D, Lo, Up, None, None, None)
     _run("rocbta_dddbtasci", _lib.OP_DDBTASCI, D, Lo, Up, None, None, None, n, b, 0, dev, True,
-         direction, "ddbtsci", B)
+         direction, "ddbtsci", B, bufs)

@@ -92,8 +92,14 @@ GemmTerm term(const double* A, int64_t lda, bool tA, const double* B, int64_t ld
   return t;
 }
 
+// A problem holds at most two terms; a third marks it invalid (nterms = -1)
+// and run_gemms rejects the list instead of launching it.
 void push_term(GemmProblem& p, const GemmTerm& t) {
   if (t.K <= 0) return;
+  if (p.nterms < 0 || p.nterms >= 2) {
+    p.nterms = -1;
+    return;
+  }
   p.t[p.nterms++] = t;
 }
 
@@ -155,6 +161,8 @@ int gemm_ts(const GemmList& list) {
 }
 
 int run_gemms(GemmList& list, hipStream_t s) {
+  for (const GemmProblem& p : list.probs)
+    if (p.nterms < 0) return ROCBTA_ERR_ARG;
   if (g_rec) {
     Phase& ph = g_rec->next();
     for (GemmProblem p : list.probs) {
@@ -2176,14 +2184,26 @@ struct Gq {  // one grouped launch, built problem by problem
 };
 }  // namespace
 
-int rocbta_dddbtasc_quadratic(int64_t n, int64_t b, int64_t a, double* D, double* Lo, double* Up,
-                              double* Rl, double* Ru, double* T, double* BD, double* BLo,
-                              double* BUp, double* BRl, double* BRu, double* BT,
-                              int invert_last_block, void* workspace, size_t workspace_bytes,
-                              void* stream) {
-  if (bad_shape(n, b, a) || !D || !BD || (n > 1 && (!Lo || !Up || !BLo || !BUp)) ||
-      (a > 0 && (!Rl || !Ru || !T || !BRl || !BRu || !BT)))
-    return ROCBTA_ERR_ARG;
+// Block views with a signed stride: the upward BT forms run the downward
+// recurrences on the block-reversed matrix J A J (J reverses the block
+// order), whose diagonal blocks are A's in reverse order and whose lower
+// (upper) blocks are A's upper (lower) blocks in reverse order; B likewise.
+// The results are the same blocks of A^{-1} and A^{-1} B A^{-T}.
+struct Bv {
+  double* p;
+  int64_t s;
+  double* operator()(int64_t i) const { return p + i * s; }
+};
+
+static Bv fwd_view(double* p, int64_t bb) { return {p, bb}; }
+static Bv rev_view(double* p, int64_t count, int64_t bb) {
+  return count > 0 ? Bv{p + (count - 1) * bb, -bb} : Bv{p, bb};
+}
+
+static int dd_quadratic_sc(int64_t n, int64_t b, int64_t a, Bv Dv, Bv Lov, Bv Upv,
+                           double* Rl, double* Ru, double* T, Bv BDv, Bv BLov, Bv BUpv,
+                           double* BRl, double* BRu, double* BT, int invert_last_block,
+                           void* workspace, size_t workspace_bytes, void* stream) {
   const DdLayout lay = dd_layout(b, a);
   if (!workspace || workspace_bytes < lay.total) return ROCBTA_ERR_WORKSPACE;
   hipStream_t s = (hipStream_t)stream;
@@ -2191,19 +2211,19 @@ int rocbta_dddbtasc_quadratic(int64_t n, int64_t b, int64_t a, double* D, double
   auto P = [&](size_t off) { return (double*)(ws + off); };
   double *W = P(lay.w), *V = P(lay.v), *Z = P(lay.z), *P1 = P(lay.p1), *P2 = P(lay.p2);
   double *M1 = P(lay.m1), *M2 = P(lay.m2), *Y1 = P(lay.bbt);
-  const int64_t bb = b * b, ab = a * b;
+  const int64_t ab = a * b;
   int rc = ROCBTA_OK;
   for (int64_t i = 1; i < n; ++i) {  // ddbtasc.py:351-445
-    double* Di = D + (i - 1) * bb;
-    const double* L = Lo + (i - 1) * bb;
-    const double* U = Up + (i - 1) * bb;
+    double* Di = Dv(i - 1);
+    const double* L = Lov(i - 1);
+    const double* U = Upv(i - 1);
     const double* R = Rl + (i - 1) * ab;
     const double* Rur = Ru + (i - 1) * ab;
     rc = block_inverse(Di, b, (int)b, ws, lay, s);
     if (rc) return rc;
     Gq g1;
     g1.p(W, b, b, b, 0.0).t(Di, b, false, U, b, false, b, 1.0);
-    g1.p(Z, b, b, b, 0.0).t(Di, b, false, BD + (i - 1) * bb, b, false, b, 1.0);
+    g1.p(Z, b, b, b, 0.0).t(Di, b, false, BDv(i - 1), b, false, b, 1.0);
     g1.p(P1, b, b, b, 0.0).t(L, b, false, Di, b, false, b, 1.0);
     if (a > 0) {
       g1.p(V, a, b, a, 0.0).t(Di, b, false, Rur, a, false, b, 1.0);
@@ -2211,25 +2231,25 @@ int rocbta_dddbtasc_quadratic(int64_t n, int64_t b, int64_t a, double* D, double
     }
     if ((rc = g1.run(s))) return rc;
     Gq g2;  // A's Schur updates; B_{i-1} <- A_{i-1}^{-1} B_{i-1} A_{i-1}^{-T}
-    g2.p(D + i * bb, b, b, b, 1.0).t(L, b, false, W, b, false, b, -1.0);
-    g2.p(BD + (i - 1) * bb, b, b, b, 0.0).t(Z, b, false, Di, b, true, b, 1.0);
+    g2.p(Dv(i), b, b, b, 1.0).t(L, b, false, W, b, false, b, -1.0);
+    g2.p(BDv(i - 1), b, b, b, 0.0).t(Z, b, false, Di, b, true, b, 1.0);
     if (a > 0) {
       g2.p(Rl + i * ab, b, a, b, 1.0).t(R, b, false, W, b, false, b, -1.0);
       g2.p(Ru + i * ab, a, b, a, 1.0).t(L, b, false, V, a, false, b, -1.0);
       g2.p(T, a, a, a, 1.0).t(R, b, false, V, a, false, b, -1.0);
     }
     if ((rc = g2.run(s))) return rc;
-    const double* Bp = BD + (i - 1) * bb;
+    const double* Bp = BDv(i - 1);
     Gq g3;
     g3.p(M1, b, b, b, 0.0).t(L, b, false, Bp, b, false, b, 1.0);
     if (a > 0) g3.p(M2, b, a, b, 0.0).t(R, b, false, Bp, b, false, b, 1.0);
     if ((rc = g3.run(s))) return rc;
-    const double* BLp = BLo + (i - 1) * bb;
-    const double* BUp_ = BUp + (i - 1) * bb;
+    const double* BLp = BLov(i - 1);
+    const double* BUp_ = BUpv(i - 1);
     const double* BRlp = BRl + (i - 1) * ab;
     const double* BRup = BRu + (i - 1) * ab;
     Gq g4;  // + M L^T - B_lower P^T
-    g4.p(BD + i * bb, b, b, b, 1.0).t(M1, b, false, L, b, true, b, 1.0).t(BLp, b, false, P1, b, true, b, -1.0);
+    g4.p(BDv(i), b, b, b, 1.0).t(M1, b, false, L, b, true, b, 1.0).t(BLp, b, false, P1, b, true, b, -1.0);
     if (a > 0) {
       g4.p(BRu + i * ab, a, b, a, 1.0).t(M1, b, false, R, b, true, b, 1.0).t(BLp, b, false, P2, b, true, b, -1.0);
       g4.p(BRl + i * ab, b, a, b, 1.0).t(M2, b, false, L, b, true, b, 1.0).t(BRlp, b, false, P1, b, true, b, -1.0);
@@ -2237,7 +2257,7 @@ int rocbta_dddbtasc_quadratic(int64_t n, int64_t b, int64_t a, double* D, double
     }
     if ((rc = g4.run(s))) return rc;
     Gq g5;  // - P B_upper
-    g5.p(BD + i * bb, b, b, b, 1.0).t(P1, b, false, BUp_, b, false, b, -1.0);
+    g5.p(BDv(i), b, b, b, 1.0).t(P1, b, false, BUp_, b, false, b, -1.0);
     if (a > 0) {
       g5.p(BRu + i * ab, a, b, a, 1.0).t(P1, b, false, BRup, a, false, b, -1.0);
       g5.p(BRl + i * ab, b, a, b, 1.0).t(P2, b, false, BUp_, b, false, b, -1.0);
@@ -2246,8 +2266,8 @@ int rocbta_dddbtasc_quadratic(int64_t n, int64_t b, int64_t a, double* D, double
     if ((rc = g5.run(s))) return rc;
   }
   if (!invert_last_block) return ROCBTA_OK;
-  double* Dl = D + (n - 1) * bb;  // ddbtasc.py:447-475
-  double* Bl = BD + (n - 1) * bb;
+  double* Dl = Dv(n - 1);  // ddbtasc.py:447-475
+  double* Bl = BDv(n - 1);
   rc = block_inverse(Dl, b, (int)b, ws, lay, s);
   if (rc) return rc;
   Gq h1;
@@ -2280,14 +2300,10 @@ int rocbta_dddbtasc_quadratic(int64_t n, int64_t b, int64_t a, double* D, double
   return h7.run(s);
 }
 
-int rocbta_dddbtasci_quadratic(int64_t n, int64_t b, int64_t a, double* D, double* Lo,
-                               double* Up, double* Rl, double* Ru, double* T, double* BD,
-                               double* BLo, double* BUp, double* BRl, double* BRu, double* BT,
-                               int invert_last_block, void* workspace, size_t workspace_bytes,
-                               void* stream) {
-  if (bad_shape(n, b, a) || !D || !BD || (n > 1 && (!Lo || !Up || !BLo || !BUp)) ||
-      (a > 0 && (!Rl || !Ru || !T || !BRl || !BRu || !BT)))
-    return ROCBTA_ERR_ARG;
+static int dd_quadratic_sci(int64_t n, int64_t b, int64_t a, Bv Dv, Bv Lov, Bv Upv,
+                            double* Rl, double* Ru, double* T, Bv BDv, Bv BLov, Bv BUpv,
+                            double* BRl, double* BRu, double* BT, int invert_last_block,
+                            void* workspace, size_t workspace_bytes, void* stream) {
   const DdLayout lay = dd_layout(b, a);
   if (!workspace || workspace_bytes < lay.total) return ROCBTA_ERR_WORKSPACE;
   hipStream_t s = (hipStream_t)stream;
@@ -2301,10 +2317,10 @@ int rocbta_dddbtasci_quadratic(int64_t n, int64_t b, int64_t a, double* D, doubl
   const int64_t bb = b * b, ab = a * b;
   int rc = ROCBTA_OK;
   if (invert_last_block && a > 0) {  // ddbtasci.py:377-430
-    double* Ad = D + (n - 1) * bb;
+    double* Ad = Dv(n - 1);
     double* Rul = Ru + (n - 1) * ab;
     double* Rll = Rl + (n - 1) * ab;
-    double* Bd = BD + (n - 1) * bb;
+    double* Bd = BDv(n - 1);
     double* t13 = BRu + (n - 1) * ab;
     double* t31 = BRl + (n - 1) * ab;
     Gq m1;
@@ -2342,21 +2358,21 @@ int rocbta_dddbtasci_quadratic(int64_t n, int64_t b, int64_t a, double* D, doubl
     if ((rc = m7.run(s))) return rc;
   }
   for (int64_t i = n - 2; i >= 0; --i) {  // ddbtasci.py:432-576
-    double* Di = D + i * bb;
-    const double* D1 = D + (i + 1) * bb;
-    double* Li = Lo + i * bb;
-    double* Ui = Up + i * bb;
+    double* Di = Dv(i);
+    const double* D1 = Dv(i + 1);
+    double* Li = Lov(i);
+    double* Ui = Upv(i);
     double* Rli = Rl + i * ab;
     double* Rui = Ru + i * ab;
     const double* Rl1 = Rl + (i + 1) * ab;
     const double* Ru1 = Ru + (i + 1) * ab;
-    double* Bi = BD + i * bb;
-    const double* Bd1 = BD + (i + 1) * bb;
+    double* Bi = BDv(i);
+    const double* Bd1 = BDv(i + 1);
     const double* Bl1 = BRl + (i + 1) * ab;
     const double* Bu1 = BRu + (i + 1) * ab;
-    double* b12 = BUp + i * bb;
+    double* b12 = BUpv(i);
     double* b13 = BRu + i * ab;
-    double* b21 = BLo + i * bb;
+    double* b21 = BLov(i);
     double* b31 = BRl + i * ab;
     RB_CHECK(hipMemcpyAsync(Dc, Di, bb * sizeof(double), hipMemcpyDeviceToDevice, s));
     RB_CHECK(hipMemcpyAsync(Bdc, Bi, bb * sizeof(double), hipMemcpyDeviceToDevice, s));
@@ -2417,6 +2433,191 @@ int rocbta_dddbtasci_quadratic(int64_t n, int64_t b, int64_t a, double* D, doubl
     lj.p(Di, b, b, b, 1.0).t(Gm, b, false, Dc, b, false, b, 1.0);
     if ((rc = lj.run(s))) return rc;
   }
+  return ROCBTA_OK;
+}
+
+static int dd_quadratic_args(int64_t n, int64_t b, int64_t a, double* D, double* Lo, double* Up,
+                             double* Rl, double* Ru, double* T, double* BD, double* BLo,
+                             double* BUp, double* BRl, double* BRu, double* BT, int direction) {
+  if (bad_shape(n, b, a) || !D || !BD || (n > 1 && (!Lo || !Up || !BLo || !BUp)) ||
+      (a > 0 && (!Rl || !Ru || !T || !BRl || !BRu || !BT)))
+    return ROCBTA_ERR_ARG;
+  if (direction != 0 && (direction != 1 || a > 0)) return ROCBTA_ERR_ARG;  // upward: BT only
+  return ROCBTA_OK;
+}
+
+int rocbta_dddbtasc_quadratic(int64_t n, int64_t b, int64_t a, double* D, double* Lo, double* Up,
+                              double* Rl, double* Ru, double* T, double* BD, double* BLo,
+                              double* BUp, double* BRl, double* BRu, double* BT,
+                              int invert_last_block, int direction, void* workspace,
+                              size_t workspace_bytes, void* stream) {
+  int rc = dd_quadratic_args(n, b, a, D, Lo, Up, Rl, Ru, T, BD, BLo, BUp, BRl, BRu, BT, direction);
+  if (rc) return rc;
+  const int64_t bb = b * b;
+  if (direction == 1)  // ddbtsc.py:314-356 as the downward sweep of J A J
+    return dd_quadratic_sc(n, b, a, rev_view(D, n, bb), rev_view(Up, n - 1, bb),
+                           rev_view(Lo, n - 1, bb), Rl, Ru, T, rev_view(BD, n, bb),
+                           rev_view(BUp, n - 1, bb), rev_view(BLo, n - 1, bb), BRl, BRu, BT,
+                           invert_last_block, workspace, workspace_bytes, stream);
+  return dd_quadratic_sc(n, b, a, fwd_view(D, bb), fwd_view(Lo, bb), fwd_view(Up, bb), Rl, Ru, T,
+                         fwd_view(BD, bb), fwd_view(BLo, bb), fwd_view(BUp, bb), BRl, BRu, BT,
+                         invert_last_block, workspace, workspace_bytes, stream);
+}
+
+int rocbta_dddbtasci_quadratic(int64_t n, int64_t b, int64_t a, double* D, double* Lo,
+                               double* Up, double* Rl, double* Ru, double* T, double* BD,
+                               double* BLo, double* BUp, double* BRl, double* BRu, double* BT,
+                               int invert_last_block, int direction, void* workspace,
+                               size_t workspace_bytes, void* stream) {
+  int rc = dd_quadratic_args(n, b, a, D, Lo, Up, Rl, Ru, T, BD, BLo, BUp, BRl, BRu, BT, direction);
+  if (rc) return rc;
+  const int64_t bb = b * b;
+  if (direction == 1)  // ddbtsci.py:372-445 as the downward selected inversion of J A J
+    return dd_quadratic_sci(n, b, a, rev_view(D, n, bb), rev_view(Up, n - 1, bb),
+                            rev_view(Lo, n - 1, bb), Rl, Ru, T, rev_view(BD, n, bb),
+                            rev_view(BUp, n - 1, bb), rev_view(BLo, n - 1, bb), BRl, BRu, BT,
+                            invert_last_block, workspace, workspace_bytes, stream);
+  return dd_quadratic_sci(n, b, a, fwd_view(D, bb), fwd_view(Lo, bb), fwd_view(Up, bb), Rl, Ru,
+                          T, fwd_view(BD, bb), fwd_view(BLo, bb), fwd_view(BUp, bb), BRl, BRu, BT,
+                          invert_last_block, workspace, workspace_bytes, stream);
+}
+
+// Permuted ("middle partition") forms of the non-SPD sweep: the middle
+// blocks 1 .. n-2 are eliminated top-down; Lbuf[k] / Ubuf[k] carry the
+// fill-ins A~_{0,k+1} / A~_{k+1,0} of row and column 0, and blocks 0, n-1
+// and the tip are left as the reduced system.
+static int dd_permuted_args(int64_t n, int64_t b, int64_t a, double* D, double* Lo, double* Up,
+                            double* Rl, double* Ru, double* T, double* Lbuf, double* Ubuf) {
+  if (bad_shape(n, b, a) || n < 2 || !D || !Lo || !Up || !Lbuf || !Ubuf ||
+      (a > 0 && (!Rl || !Ru || !T)))
+    return ROCBTA_ERR_ARG;
+  return ROCBTA_OK;
+}
+
+int rocbta_dddbtasc_permuted(int64_t n, int64_t b, int64_t a, double* D, double* Lo, double* Up,
+                             double* Rl, double* Ru, double* T, double* Lbuf, double* Ubuf,
+                             void* workspace, size_t workspace_bytes, void* stream) {
+  int rc = dd_permuted_args(n, b, a, D, Lo, Up, Rl, Ru, T, Lbuf, Ubuf);
+  if (rc) return rc;
+  const DdLayout lay = dd_layout(b, a);
+  if (!workspace || workspace_bytes < lay.total) return ROCBTA_ERR_WORKSPACE;
+  hipStream_t s = (hipStream_t)stream;
+  char* ws = (char*)workspace;
+  double* W = (double*)(ws + lay.w);
+  double* V = (double*)(ws + lay.v);
+  double* Y = (double*)(ws + lay.z);
+  const int64_t bb = b * b, ab = a * b;
+  // ddbtasc.py:256-257: the fill-ins start as A_{0,1} and A_{1,0}
+  RB_CHECK(hipMemcpyAsync(Lbuf, Up, bb * sizeof(double), hipMemcpyDeviceToDevice, s));
+  RB_CHECK(hipMemcpyAsync(Ubuf, Lo, bb * sizeof(double), hipMemcpyDeviceToDevice, s));
+  for (int64_t i = 1; i + 1 < n; ++i) {  // ddbtasc.py:259-332
+    double* S = D + i * bb;
+    const double* L = Lo + i * bb;
+    const double* R = Rl + i * ab;
+    const double* Lp = Lbuf + (i - 1) * bb;
+    rc = block_inverse(S, b, (int)b, ws, lay, s);
+    if (rc) return rc;
+    // W = S^-1 A_{i,i+1}, Y = S^-1 A~_{i,0}, V = S^-1 A_{i,t}
+    Gq g1;
+    g1.p(W, b, b, b, 0.0).t(S, b, false, Up + i * bb, b, false, b, 1.0);
+    g1.p(Y, b, b, b, 0.0).t(S, b, false, Ubuf + (i - 1) * bb, b, false, b, 1.0);
+    if (a > 0) g1.p(V, a, b, a, 0.0).t(S, b, false, Ru + i * ab, a, false, b, 1.0);
+    if ((rc = g1.run(s))) return rc;
+    // every update reads only blocks this step does not write: one launch
+    Gq g2;
+    g2.p(D + (i + 1) * bb, b, b, b, 1.0).t(L, b, false, W, b, false, b, -1.0);
+    g2.p(Lbuf + i * bb, b, b, b, 0.0).t(Lp, b, false, W, b, false, b, -1.0);
+    g2.p(Ubuf + i * bb, b, b, b, 0.0).t(L, b, false, Y, b, false, b, -1.0);
+    g2.p(D, b, b, b, 1.0).t(Lp, b, false, Y, b, false, b, -1.0);
+    if (a > 0) {
+      g2.p(Rl + (i + 1) * ab, b, a, b, 1.0).t(R, b, false, W, b, false, b, -1.0);
+      g2.p(Ru + (i + 1) * ab, a, b, a, 1.0).t(L, b, false, V, a, false, b, -1.0);
+      g2.p(T, a, a, a, 1.0).t(R, b, false, V, a, false, b, -1.0);
+      g2.p(Rl, b, a, b, 1.0).t(R, b, false, Y, b, false, b, -1.0);
+      g2.p(Ru, a, b, a, 1.0).t(Lp, b, false, V, a, false, b, -1.0);
+    }
+    if ((rc = g2.run(s))) return rc;
+  }
+  return ROCBTA_OK;
+}
+
+int rocbta_dddbtasci_permuted(int64_t n, int64_t b, int64_t a, double* D, double* Lo, double* Up,
+                              double* Rl, double* Ru, double* T, double* Lbuf, double* Ubuf,
+                              void* workspace, size_t workspace_bytes, void* stream) {
+  int rc = dd_permuted_args(n, b, a, D, Lo, Up, Rl, Ru, T, Lbuf, Ubuf);
+  if (rc) return rc;
+  const DdLayout lay = dd_layout(b, a);
+  if (!workspace || workspace_bytes < lay.total) return ROCBTA_ERR_WORKSPACE;
+  hipStream_t s = (hipStream_t)stream;
+  char* ws = (char*)workspace;
+  auto P = [&](size_t off) { return (double*)(ws + off); };
+  double *B1 = P(lay.b1), *B2 = P(lay.sm), *B3 = P(lay.b2), *C1 = P(lay.c1), *C2 = P(lay.mt);
+  double *C3 = P(lay.c2), *E = P(lay.e), *Gm = P(lay.g), *Dc = P(lay.dc);
+  const int64_t bb = b * b, ab = a * b;
+  for (int64_t i = n - 2; i >= 1; --i) {  // ddbtasci.py:280-339
+    double* Di = D + i * bb;
+    const double* D1 = D + (i + 1) * bb;
+    double* Li = Lo + i * bb;
+    double* Ui = Up + i * bb;
+    double* Rli = Rl + i * ab;
+    double* Rui = Ru + i * ab;
+    const double* Rl1 = Rl + (i + 1) * ab;
+    const double* Ru1 = Ru + (i + 1) * ab;
+    const double* Lb1 = Lbuf + i * bb;        // X_{0,i+1}
+    const double* Ub1 = Ubuf + i * bb;        // X_{i+1,0}
+    double* Lb0 = Lbuf + (i - 1) * bb;        // A~_{0,i} -> X_{0,i}
+    double* Ub0 = Ubuf + (i - 1) * bb;        // A~_{i,0} -> X_{i,0}
+    RB_CHECK(hipMemcpyAsync(Dc, Di, bb * sizeof(double), hipMemcpyDeviceToDevice, s));
+    // B: row i of A~ times the finished columns; C: finished rows times column i
+    const bool ar = a > 0;
+    // B: row i of A~ times the finished columns; C: the finished rows times
+    // column i.  Two terms per launch: the third (arrow) term follows.
+    Gq g1;
+    g1.p(B1, b, b, b, 0.0).t(Ui, b, false, D1, b, false, b, 1.0).t(Ub0, b, false, Lb1, b, false, b, 1.0);
+    g1.p(B2, b, b, b, 0.0).t(Ui, b, false, Ub1, b, false, b, 1.0).t(Ub0, b, false, D, b, false, b, 1.0);
+    g1.p(C1, b, b, b, 0.0).t(D1, b, false, Li, b, false, b, 1.0).t(Ub1, b, false, Lb0, b, false, b, 1.0);
+    g1.p(C2, b, b, b, 0.0).t(Lb1, b, false, Li, b, false, b, 1.0).t(D, b, false, Lb0, b, false, b, 1.0);
+    if (ar) {
+      g1.p(B3, a, b, a, 0.0).t(Ui, b, false, Ru1, a, false, b, 1.0).t(Rui, a, false, T, a, false, a, 1.0);
+      g1.p(C3, b, a, b, 0.0).t(Rl1, b, false, Li, b, false, b, 1.0).t(T, a, false, Rli, b, false, a, 1.0);
+    }
+    if ((rc = g1.run(s))) return rc;
+    if (ar) {
+      Gq h1;
+      h1.p(B1, b, b, b, 1.0).t(Rui, a, false, Rl1, b, false, a, 1.0);
+      h1.p(B2, b, b, b, 1.0).t(Rui, a, false, Rl, b, false, a, 1.0);
+      h1.p(C1, b, b, b, 1.0).t(Ru1, a, false, Rli, b, false, a, 1.0);
+      h1.p(C2, b, b, b, 1.0).t(Ru, a, false, Rli, b, false, a, 1.0);
+      h1.p(B3, a, b, a, 1.0).t(Ub0, b, false, Ru, a, false, b, 1.0);
+      h1.p(C3, b, a, b, 1.0).t(Rl, b, false, Lb0, b, false, b, 1.0);
+      if ((rc = h1.run(s))) return rc;
+    }
+    // E = B1 A_{i+1,i} + B2 A~_{0,i} + B3 A~_{t,i} (old blocks); the new row i
+    Gq g2;
+    g2.p(E, b, b, b, 0.0).t(B1, b, false, Li, b, false, b, 1.0).t(B2, b, false, Lb0, b, false, b, 1.0);
+    g2.p(Ui, b, b, b, 0.0).t(Dc, b, false, B1, b, false, b, -1.0);
+    g2.p(Ub0, b, b, b, 0.0).t(Dc, b, false, B2, b, false, b, -1.0);
+    if (ar) g2.p(Rui, a, b, a, 0.0).t(Dc, b, false, B3, a, false, b, -1.0);
+    if ((rc = g2.run(s))) return rc;
+    if (ar) {
+      Gq h2;
+      h2.p(E, b, b, b, 1.0).t(B3, a, false, Rli, b, false, a, 1.0);
+      if ((rc = h2.run(s))) return rc;
+    }
+    // the new column i, and Gm = S^-1 E
+    Gq g3;
+    g3.p(Li, b, b, b, 0.0).t(C1, b, false, Dc, b, false, b, -1.0);
+    g3.p(Lb0, b, b, b, 0.0).t(C2, b, false, Dc, b, false, b, -1.0);
+    if (a > 0) g3.p(Rli, b, a, b, 0.0).t(C3, b, false, Dc, b, false, b, -1.0);
+    g3.p(Gm, b, b, b, 0.0).t(Dc, b, false, E, b, false, b, 1.0);
+    if ((rc = g3.run(s))) return rc;
+    Gq g4;  // X_ii = S^-1 + S^-1 E S^-1
+    g4.p(Di, b, b, b, 1.0).t(Gm, b, false, Dc, b, false, b, 1.0);
+    if ((rc = g4.run(s))) return rc;
+  }
+  // ddbtasci.py:340-341: X_{1,0}, X_{0,1} back to the tridiagonal storage
+  RB_CHECK(hipMemcpyAsync(Lo, Ubuf, bb * sizeof(double), hipMemcpyDeviceToDevice, s));
+  RB_CHECK(hipMemcpyAsync(Up, Lbuf, bb * sizeof(double), hipMemcpyDeviceToDevice, s));
   return ROCBTA_OK;
 }
 

@@ -146,13 +146,32 @@ def test_ddbt_workspace_and_argument_errors(lib):
                                 None) == _lib.ROCBTA_ERR_WORKSPACE
 
 
+def test_ddbt_permuted_argument_errors(lib):
+    fake = ctypes.c_void_p(16)
+    big = 1 << 24
+    # n < 2, a missing buffer, then a workspace too small
+    assert lib.rocbta_dddbtasc_permuted(1, 4, 2, *([fake] * 8), fake, big,
+                                        None) == _lib.ROCBTA_ERR_ARG
+    assert lib.rocbta_dddbtasci_permuted(3, 4, 2, *([fake] * 7), None, fake, big,
+                                         None) == _lib.ROCBTA_ERR_ARG
+    assert lib.rocbta_dddbtasc_permuted(3, 4, 0, fake, fake, fake, None, None, None, fake, fake,
+                                        fake, 8, None) == _lib.ROCBTA_ERR_WORKSPACE
+
+
 def test_ddbt_quadratic_argument_errors(lib):
     fake = ctypes.c_void_p(16)
     big = 1 << 26
     # missing B arrays, then a workspace too small
     assert lib.rocbta_dddbtasc_quadratic(3, 4, 2, fake, fake, fake, fake, fake, fake, None, fake,
-                                         fake, fake, fake, fake, 1, fake, big, None) == _lib.ROCBTA_ERR_ARG
+                                         fake, fake, fake, fake, 1, 0, fake, big, None) == _lib.ROCBTA_ERR_ARG
     assert lib.rocbta_dddbtasci_quadratic(3, 4, 2, fake, fake, fake, fake, fake, fake, fake, fake,
-                                          fake, fake, None, fake, 1, fake, big, None) == _lib.ROCBTA_ERR_ARG
-    assert lib.rocbta_dddbtasci_quadratic(3, 4, 2, *([fake] * 12), 1, fake, 8,
+                                          fake, fake, None, fake, 1, 0, fake, big, None) == _lib.ROCBTA_ERR_ARG
+    assert lib.rocbta_dddbtasci_quadratic(3, 4, 2, *([fake] * 12), 1, 0, fake, 8,
+                                          None) == _lib.ROCBTA_ERR_WORKSPACE
+    # upward: the BT form only; an unknown direction
+    assert lib.rocbta_dddbtasc_quadratic(3, 4, 2, *([fake] * 12), 1, 1, fake, big,
+                                         None) == _lib.ROCBTA_ERR_ARG
+    assert lib.rocbta_dddbtasci_quadratic(3, 4, 0, *([fake] * 12), 1, 2, fake, big,
+                                          None) == _lib.ROCBTA_ERR_ARG
+    assert lib.rocbta_dddbtasci_quadratic(3, 4, 0, *([fake] * 12), 1, 1, fake, 8,
                                           None) == _lib.ROCBTA_ERR_WORKSPACE

@@ -12,7 +12,8 @@ Shapes: the reference grid (b, a in {2, 3}, n 1..4), 64-wide tile
 boundaries of the blocked inverse (b 63, 64, 65, 100, 130, 256; a 1, 16,
 64, 65), the arrowless forms in both directions; the A X A^T = B forms
 (symmetric B, as the reference's tests) against the oracle and against
-A^{-1} B A^{-T} from dense linear algebra.
+A^{-1} B A^{-T} from dense linear algebra, the BT ones in both directions
+(ddbtsc.py:271-356, ddbtsci.py:299-445).
 """
 
 import numpy as np
@@ -110,9 +111,9 @@ def test_ddbt_argument_errors(algs, gpu_device):
     with pytest.raises(NotImplementedError, match="AX=B"):
         algs.ddbtasc(*g, rhs=rhs)
     bt = {k: rhs[k] for k in list(rhs)[:3]}
-    with pytest.raises(NotImplementedError, match="upward"):
-        algs.ddbtsc(g[0], g[1], g[2], rhs=bt, quadratic=True, direction="upward")
-    with pytest.raises(NotImplementedError, match="buffers"):
+    with pytest.raises(ValueError, match="direction"):
+        algs.ddbtsc(g[0], g[1], g[2], rhs=bt, quadratic=True, direction="sideways")
+    with pytest.raises(ValueError, match="buffers does not contain"):
         algs.ddbtasci(*g, buffers={})
     with pytest.raises(ValueError, match="direction"):
         algs.ddbtsc(g[0], g[1], g[2], direction="sideways")
@@ -156,5 +157,34 @@ def test_ddbt_quadratic_matches_oracle_and_dense(algs, gpu_device, b, a, n):
     X = np.linalg.inv(A)
     ref = list(dense.bta_dense_to_arrays_full(X, b, a, n)) + list(
         dense.bta_dense_to_arrays_full(X @ Bm @ X.T, b, a, n))
+    for x, y in zip(h, ref):
+        assert relerr(x, y) < TOL
+
+
+@pytest.mark.parametrize("direction", ["downward", "upward"])
+@pytest.mark.parametrize("b,n", [(2, 1), (3, 4), (63, 3), (64, 5), (100, 4), (130, 2)])
+def test_ddbtsc_quadratic_both_directions(algs, gpu_device, b, n, direction):
+    print(f"ddbtsc quadratic {direction} b={b} n={n}", flush=True)
+    rng = np.random.default_rng(45000 + b + 13 * n + (7 if direction == "upward" else 0))
+    A = generators.dd_bt(b, n, rng)
+    Bm = generators.dd_bt(b, n, rng)
+    generators.symmetrize(Bm)
+    arrs = dense.bta_dense_to_arrays_full(A, b, 0, n)[:3]
+    brr = dense.bta_dense_to_arrays_full(Bm, b, 0, n)[:3]
+    g, gb = to_dev(gpu_device, *arrs), to_dev(gpu_device, *brr)
+    rhs = dict(zip(("B_diagonal_blocks", "B_lower_diagonal_blocks", "B_upper_diagonal_blocks"), gb))
+    algs.ddbtsc(*g, rhs=rhs, quadratic=True, direction=direction)
+    o, ob = [x.copy() for x in arrs], [x.copy() for x in brr]
+    serinv_np.ddbtsc_quadratic(*o, *ob, direction=direction)
+    for x, y in zip(to_host(*g, *gb), o + ob):
+        assert relerr(x, y) < TOL
+    algs.ddbtsci(*g, rhs=rhs, quadratic=True, direction=direction)
+    serinv_np.ddbtsci_quadratic(*o, *ob, direction=direction)
+    h = to_host(*g, *gb)
+    for x, y in zip(h, o + ob):
+        assert relerr(x, y) < TOL
+    X = np.linalg.inv(A)
+    ref = list(dense.bta_dense_to_arrays_full(X, b, 0, n)[:3]) + list(
+        dense.bta_dense_to_arrays_full(X @ Bm @ X.T, b, 0, n)[:3])
     for x, y in zip(h, ref):
         assert relerr(x, y) < TOL

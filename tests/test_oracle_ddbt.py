@@ -150,3 +150,79 @@ def test_ddbtasc_ddbtasci_quadratic_vs_dense(b, a, n):
     for x, y in zip(brr, dense.bta_dense_to_arrays_full(XBX, b, a, n)):
         assert relerr(x, y) < RTOL
 
+
+
+@pytest.mark.parametrize("direction", ["downward", "upward"])
+@pytest.mark.parametrize("b,n", [(b, n) for b in (2, 3) for n in (1, 2, 3, 4, 5)])
+def test_ddbtsc_ddbtsci_quadratic_vs_dense(b, n, direction):
+    """BT A X A^T = B in both directions (test_ddbtsc.py / test_ddbtsci.py,
+    "AXA.T=B" x {downward, upward}): the block each sweep ends on holds
+    X_ee and (X B X^T)_ee; after the selected inversion every block on the
+    pattern is X's and X B X^T's."""
+    rng = np.random.default_rng(37000 + 10 * b + n + (500 if direction == "upward" else 0))
+    A = generators.dd_bt(b, n, rng)
+    B = generators.dd_bt(b, n, rng)
+    generators.symmetrize(B)
+    X = np.linalg.inv(A)
+    XBX = X @ B @ X.T
+    D, Lo, Up, _, _, _ = dense.bta_dense_to_arrays_full(A, b, 0, n)
+    BD, BLo, BUp, _, _, _ = dense.bta_dense_to_arrays_full(B, b, 0, n)
+    serinv_np.ddbtsc_quadratic(D, Lo, Up, BD, BLo, BUp, direction=direction)
+    e = slice((n - 1) * b, n * b) if direction == "downward" else slice(0, b)
+    end = -1 if direction == "downward" else 0
+    assert relerr(D[end], X[e, e]) < RTOL
+    assert relerr(BD[end], XBX[e, e]) < RTOL
+    serinv_np.ddbtsci_quadratic(D, Lo, Up, BD, BLo, BUp, direction=direction)
+    for x, y in zip((D, Lo, Up), dense.bta_dense_to_arrays_full(X, b, 0, n)[:3]):
+        assert relerr(x, y) < RTOL
+    for x, y in zip((BD, BLo, BUp), dense.bta_dense_to_arrays_full(XBX, b, 0, n)[:3]):
+        assert relerr(x, y) < RTOL
+
+
+@pytest.mark.parametrize("b,n", [(3, 4), (5, 6)])
+def test_ddbtsc_quadratic_upward_is_reversed_downward(b, n):
+    """The upward sweep on A equals the downward sweep on the block-reversed
+    matrix J A J, block for block (lower and upper blocks trade places);
+    the selected inversions agree the same way.  Holds for the A and B
+    arrays at every intermediate state, so a wrong index or a swapped
+    lower/upper block in either direction fails here."""
+    rng = np.random.default_rng(38000 + b + n)
+    A = generators.dd_bt(b, n, rng)
+    B = generators.dd_bt(b, n, rng)
+    generators.symmetrize(B)
+    up = [x.copy() for x in dense.bta_dense_to_arrays_full(A, b, 0, n)[:3]] + \
+        [x.copy() for x in dense.bta_dense_to_arrays_full(B, b, 0, n)[:3]]
+    dn = [up[0][::-1].copy(), up[2][::-1].copy(), up[1][::-1].copy(),
+          up[3][::-1].copy(), up[5][::-1].copy(), up[4][::-1].copy()]
+    serinv_np.ddbtsc_quadratic(*up, direction="upward")
+    serinv_np.ddbtsc_quadratic(*dn, direction="downward")
+    for x, y in zip(up, (dn[0][::-1], dn[2][::-1], dn[1][::-1], dn[3][::-1], dn[5][::-1],
+                         dn[4][::-1])):
+        assert relerr(x, y) < 1e-13
+    serinv_np.ddbtsci_quadratic(*up, direction="upward")
+    serinv_np.ddbtsci_quadratic(*dn, direction="downward")
+    for x, y in zip(up, (dn[0][::-1], dn[2][::-1], dn[1][::-1], dn[3][::-1], dn[5][::-1],
+                         dn[4][::-1])):
+        assert relerr(x, y) < 1e-12
+
+
+@pytest.mark.parametrize("b,n", [(2, 3), (4, 5)])
+def test_ddbtsc_quadratic_downward_equals_arrow_form(b, n):
+    """The BT restatement (ddbtsc.py:271-311, ddbtsci.py:299-369) against the
+    BTA restatement with an empty arrow (ddbtasc.py:335-475,
+    ddbtasci.py:343-576), which the BT forms specialise."""
+    rng = np.random.default_rng(39000 + b + n)
+    A = generators.dd_bt(b, n, rng)
+    B = generators.dd_bt(b, n, rng)
+    generators.symmetrize(B)
+    bt = [x.copy() for x in dense.bta_dense_to_arrays_full(A, b, 0, n)[:3]] + \
+        [x.copy() for x in dense.bta_dense_to_arrays_full(B, b, 0, n)[:3]]
+    ar = list(dense.bta_dense_to_arrays_full(A, b, 0, n)) + list(dense.bta_dense_to_arrays_full(B, b, 0, n))
+    serinv_np.ddbtsc_quadratic(*bt)
+    serinv_np.ddbtasc_quadratic(*ar)
+    for x, y in zip(bt, ar[:3] + ar[6:9]):
+        assert relerr(x, y) < 1e-13
+    serinv_np.ddbtsci_quadratic(*bt)
+    serinv_np.ddbtasci_quadratic(*ar)
+    for x, y in zip(bt, ar[:3] + ar[6:9]):
+        assert relerr(x, y) < 1e-12

@@ -255,3 +255,57 @@ DDBT_SCI_MUTANTS = [
 @pytest.mark.parametrize("old,new", DDBT_SCI_MUTANTS)
 def test_ddbtasci_mutants_are_caught(old, new):
     assert ddbt_err(sci=mutant(serinv_np.ddbtasci, old, new)) > 1e-6
+
+
+# BT A X A^T = B, both directions: mutants must change X or X B X^T.
+def ddbt_quad_err(direction, sc=None, sci=None):
+    b, n = 3, 4
+    rng = np.random.default_rng(78)
+    A = generators.dd_bt(b, n, rng)
+    B = generators.dd_bt(b, n, rng)
+    generators.symmetrize(B)
+    arrs = list(dense.bta_dense_to_arrays_full(A, b, 0, n)[:3]) + \
+        list(dense.bta_dense_to_arrays_full(B, b, 0, n)[:3])
+    (sc or serinv_np.ddbtsc_quadratic)(*arrs, direction=direction)
+    (sci or serinv_np.ddbtsci_quadratic)(*arrs, direction=direction)
+    X = np.linalg.inv(A)
+    ref = list(dense.bta_dense_to_arrays_full(X, b, 0, n)[:3]) + \
+        list(dense.bta_dense_to_arrays_full(X @ B @ X.T, b, 0, n)[:3])
+    return max(np.max(np.abs(x - y)) / np.max(np.abs(y)) for x, y in zip(arrs, ref))
+
+
+@pytest.mark.parametrize("direction", ["downward", "upward"])
+def test_unmutated_ddbt_quadratic_oracle_passes(direction):
+    assert ddbt_quad_err(direction) < TOL
+
+
+DDBT_QSC_MUTANTS = [
+    ("upward", "BD[i - 1] + Up[i - 1] @ BD[i] @ Up[i - 1].T - BUp[i - 1] @ UD.T - UD @ BLo[i - 1]",
+     "BD[i - 1] + Up[i - 1] @ BD[i] @ Up[i - 1].T - BLo[i - 1] @ UD.T - UD @ BUp[i - 1]"),
+    ("upward", "D[i - 1] = D[i - 1] - UD @ Lo[i - 1]", "D[i - 1] = D[i - 1] - UD @ Up[i - 1]"),
+    ("upward", "BD[last] = D[last] @ BD[last] @ D[last].T", "BD[last] = D[last] @ BD[last]"),
+    ("downward", "- BLo[i] @ LD.T - LD @ BUp[i]", "- BLo[i] @ LD.T"),
+]
+
+
+@pytest.mark.parametrize("direction,old,new", DDBT_QSC_MUTANTS)
+def test_ddbtsc_quadratic_mutants_are_caught(direction, old, new):
+    bad = mutant(serinv_np.ddbtsc_quadratic, old, new)
+    assert ddbt_quad_err(direction, sc=bad) > 1e-6
+
+
+_QSCI_BOTH = [
+    ("t4 = AL[k].T @ D[f].T", "t4 = AL[k] @ D[f].T"),
+    ("+ t1 @ t3 @ BD[g] +", "+"),
+    ("BL[k] = -BD[f] @ t2 - t3 @ BD[g] + D[f] @ Bfg @ D[g].T",
+     "BL[k] = -BD[f] @ t2 - t3 @ BD[g] + D[f] @ Bgf @ D[g].T"),
+]
+DDBT_QSCI_MUTANTS = [(d, o, nw) for d in ("downward", "upward") for o, nw in _QSCI_BOTH] + [
+    ("upward", "AU, AL, BU, BL = Lo, Up, BLo, BUp", "AU, AL, BU, BL = Lo, Up, BUp, BLo"),
+]
+
+
+@pytest.mark.parametrize("direction,old,new", DDBT_QSCI_MUTANTS)
+def test_ddbtsci_quadratic_mutants_are_caught(direction, old, new):
+    bad = mutant(serinv_np.ddbtsci_quadratic, old, new)
+    assert ddbt_quad_err(direction, sci=bad) > 1e-6
