@@ -317,6 +317,27 @@ int rocbta_dddbtasci_permuted(int64_t n, int64_t b, int64_t a, double* D, double
                               double* Rl, double* Ru, double* T, double* Lbuf, double* Ubuf,
                               void* workspace, size_t workspace_bytes, void* stream);
 
+/* The permuted A X A^T = B forms: the same sweeps with B's blocks and B's
+ * fill-in buffers (BLbuf, BUbuf, like Lbuf, Ubuf) carried along; BLbuf[k],
+ * BUbuf[k] for k >= 1 accumulate and must be zero on entry (as the
+ * reference's allocator leaves them).  For a symmetric B.  Replace
+ * _ddbtasc_quadratic_permuted (ddbtasc.py:478-710) / _ddbtsc_quadratic_
+ * permuted (ddbtsc.py:357-466) and _ddbtasci_quadratic_permuted
+ * (ddbtasci.py:578-845) / _ddbtsci_quadratic_permuted (ddbtsci.py:446-639).
+ * Workspace: ROCBTA_OP_DDBTASC. */
+int rocbta_dddbtasc_quadratic_permuted(int64_t n, int64_t b, int64_t a, double* D, double* Lo,
+                                       double* Up, double* Rl, double* Ru, double* T,
+                                       double* Lbuf, double* Ubuf, double* BD, double* BLo,
+                                       double* BUp, double* BRl, double* BRu, double* BT,
+                                       double* BLbuf, double* BUbuf, void* workspace,
+                                       size_t workspace_bytes, void* stream);
+int rocbta_dddbtasci_quadratic_permuted(int64_t n, int64_t b, int64_t a, double* D, double* Lo,
+                                        double* Up, double* Rl, double* Ru, double* T,
+                                        double* Lbuf, double* Ubuf, double* BD, double* BLo,
+                                        double* BUp, double* BRl, double* BRu, double* BT,
+                                        double* BLbuf, double* BUbuf, void* workspace,
+                                        size_t workspace_bytes, void* stream);
+
 /* -------------------------------------------------------------- streaming
  * Host <-> device streaming forms for systems whose blocks do not fit in
  * device memory: D, L, R, T are HOST pointers (page-locked memory, e.g.

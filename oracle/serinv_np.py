@@ -576,3 +576,96 @@ def ddbtasci_permuted(D, Lo, Up, Rl, Ru, T, Lbuf, Ubuf):
     if n > 1:
         Lo[0] = Ubuf[0]  # X_{1,0}
         Up[0] = Lbuf[0]  # X_{0,1}
+
+
+def ddbtasc_quadratic_permuted(D, Lo, Up, Rl, Ru, T, Lbuf, Ubuf, BD, BLo, BUp, BRl, BRu, BT,
+                               BLbuf, BUbuf):
+    """Permuted sweep for A X A^T = B, src/serinv/algs/ddbtasc.py:478-710 (with
+    a = 0, ddbtsc.py:357-466): the A sweep of ddbtasc_permuted, then B's blocks
+    carried through the same elimination.  With F_r = A~_{r,i}, G_r =
+    B~_{r,i}, H_r = B~_{i,r} for the neighbours r in (i+1, t, 0) of the
+    eliminated block i, and S = A~_ii^{-1}: B~_ii <- S B~_ii S^T, then
+    B~_rc += F_r B~_ii F_c^T - G_r S^T F_c^T - F_r S H_c for every pair
+    (ddbtasc.py:581-708).  BLbuf[k], BUbuf[k] for k >= 1 accumulate, so they
+    must start at zero (the reference's allocator zeroes them)."""
+    n = D.shape[0]
+    Lbuf[0] = Up[0]  # :498-501
+    Ubuf[0] = Lo[0]
+    BLbuf[0] = BUp[0]
+    BUbuf[0] = BLo[0]
+    for i in range(1, n - 1):
+        D[i] = np.linalg.inv(D[i])
+        S = D[i]
+        D[i + 1] = D[i + 1] - Lo[i] @ S @ Up[i]
+        Rl[i + 1] = Rl[i + 1] - Rl[i] @ S @ Up[i]
+        Ru[i + 1] = Ru[i + 1] - Lo[i] @ S @ Ru[i]
+        T[:, :] = T - Rl[i] @ S @ Ru[i]
+        Lbuf[i] = -Lbuf[i - 1] @ S @ Up[i]
+        Ubuf[i] = -Lo[i] @ S @ Ubuf[i - 1]
+        D[0] = D[0] - Lbuf[i - 1] @ S @ Ubuf[i - 1]
+        Rl[0] = Rl[0] - Rl[i] @ S @ Ubuf[i - 1]
+        Ru[0] = Ru[0] - Lbuf[i - 1] @ S @ Ru[i]
+        BD[i] = S @ BD[i] @ S.T  # :581-583
+        F = {"n": Lo[i], "t": Rl[i], "0": Lbuf[i - 1]}
+        G = {"n": BLo[i], "t": BRl[i], "0": BLbuf[i - 1]}
+        H = {"n": BUp[i], "t": BRu[i], "0": BUbuf[i - 1]}
+
+        def upd(r, c):
+            return F[r] @ BD[i] @ F[c].T - G[r] @ S.T @ F[c].T - F[r] @ S @ H[c]
+
+        BD[i + 1] = BD[i + 1] + upd("n", "n")
+        BRl[i + 1] = BRl[i + 1] + upd("t", "n")
+        BRu[i + 1] = BRu[i + 1] + upd("n", "t")
+        BT[:, :] = BT + upd("t", "t")
+        BLbuf[i] = BLbuf[i] + upd("0", "n")
+        BUbuf[i] = BUbuf[i] + upd("n", "0")
+        BD[0] = BD[0] + upd("0", "0")
+        BRl[0] = BRl[0] + upd("t", "0")
+        BRu[0] = BRu[0] + upd("0", "t")
+
+
+def ddbtasci_quadratic_permuted(D, Lo, Up, Rl, Ru, T, Lbuf, Ubuf, BD, BLo, BUp, BRl, BRu, BT,
+                                BLbuf, BUbuf):
+    """Permuted selected inversion for A X A^T = B, src/serinv/algs/ddbtasci.py:
+    578-845 (with a = 0, ddbtsci.py:446-639).  Expects the reduced system's
+    inverse (A's and B's) in blocks 0, n-1, the tip, their arrow blocks and
+    the last buffer blocks.  Step i (n-2 .. 1) works with the finished blocks
+    j, k in (i+1, 0, t): X_jk and XB_jk (A^{-1} and A^{-1} B A^{-T}), A~'s row
+    i (a_j = A~_{i,j}) and column i (c_j = A~_{j,i}), and B~'s (bR_j =
+    B~_{i,j}, bC_j = B~_{j,i}); the reference's B1..B3 / C1..C3 are B_j =
+    sum_k a_k X_kj and C_j = sum_k X_jk c_k (:619-653).  For a symmetric B."""
+    n = D.shape[0]
+    for i in range(n - 2, 0, -1):
+        S, SB = D[i].copy(), BD[i].copy()
+        a = [Up[i], Ubuf[i - 1], Ru[i]]
+        c = [Lo[i], Lbuf[i - 1], Rl[i]]
+        X = [[D[i + 1], Ubuf[i], Ru[i + 1]], [Lbuf[i], D[0], Ru[0]], [Rl[i + 1], Rl[0], T]]
+        XB = [[BD[i + 1], BUbuf[i], BRu[i + 1]], [BLbuf[i], BD[0], BRu[0]],
+              [BRl[i + 1], BRl[0], BT]]
+        bR = [BUp[i].copy(), BUbuf[i - 1].copy(), BRu[i].copy()]
+        bC = [BLo[i].copy(), BLbuf[i - 1].copy(), BRl[i].copy()]
+        J = range(3)
+        Bj = [sum(a[k] @ X[k][j] for k in J) for j in J]
+        Cj = [sum(X[j][k] @ c[k] for k in J) for j in J]
+        DF = [sum(a[k] @ XB[k][j] for k in J) for j in J]  # :664-709
+        Gj = [sum(bR[k] @ X[j][k].T for k in J) for j in J]
+        DK = [sum(XB[j][k] @ a[k].T for k in J) for j in J]  # :710-755
+        Hj = [sum(X[j][k] @ bC[k] for k in J) for j in J]
+        newR = [-S @ DF[j] - SB @ Cj[j].T + S @ Gj[j] for j in J]
+        newC = [-DK[j] @ S.T - Cj[j] @ SB + Hj[j] @ S.T for j in J]
+        E = sum(Bj[j] @ c[j] for j in J)
+        newBD = (SB + S @ sum(DF[j] @ a[j].T for j in J) @ S.T + S @ E @ SB
+                 + SB.T @ sum(Cj[j].T @ a[j].T for j in J) @ S.T
+                 - S @ sum(Bj[j] @ bC[j] for j in J) @ S.T
+                 - S @ sum(Gj[j] @ a[j].T for j in J) @ S.T)  # :756-821
+        BUp[i], BUbuf[i - 1], BRu[i] = newR
+        BLo[i], BLbuf[i - 1], BRl[i] = newC
+        BD[i] = newBD
+        Up[i], Ubuf[i - 1], Ru[i] = (-S @ Bj[j] for j in J)  # :822-839
+        Lo[i], Lbuf[i - 1], Rl[i] = (-Cj[j] @ S for j in J)
+        D[i] = S + S @ E @ S
+    if n > 1:  # :841-845
+        Lo[0] = Ubuf[0]
+        Up[0] = Lbuf[0]
+        BLo[0] = BUbuf[0]
+        BUp[0] = BLbuf[0]

@@ -69,6 +69,10 @@ SIGNATURES = {
                                  [_c_p, _c_size, _c_p]),
     "rocbta_dddbtasci_permuted": (_c_int, [_c_i64, _c_i64, _c_i64] + [_c_p] * 8 +
                                   [_c_p, _c_size, _c_p]),
+    "rocbta_dddbtasc_quadratic_permuted": (_c_int, [_c_i64, _c_i64, _c_i64] + [_c_p] * 16 +
+                                           [_c_p, _c_size, _c_p]),
+    "rocbta_dddbtasci_quadratic_permuted": (_c_int, [_c_i64, _c_i64, _c_i64] + [_c_p] * 16 +
+                                            [_c_p, _c_size, _c_p]),
     "rocbta_dlogdet": (_c_int, [_c_i64, _c_i64, _c_i64, _c_p, _c_p, _c_p, _c_p, _c_size, _c_p]),
     "rocbta_dag_selfcheck": (_c_i64, [_c_i64, _c_i64, _c_i64]),
     "rocbta_mpobtaf": (_c_int, [_c_i64, _c_i64, _c_i64, _c_p, _c_p, _c_p, _c_p, _c_int, _c_p,

@@ -11,11 +11,13 @@ ddbtasc.py:335-475, ddbtasci.py:343-576; downward) runs as
 rocbta_dddbtasc_quadratic / rocbta_dddbtasci_quadratic; the upward BT
 quadratic forms (ddbtsc.py:314-356, ddbtsci.py:372-445) run through the
 same entry points with direction=1.  rhs= without quadratic (the
-reference's "AX=B" branch is an empty placeholder, ddbtasc.py:183-185) and
-the quadratic permuted forms raise NotImplementedError.  buffers= without
-rhs (ddbtasc.py:244-332, ddbtasci.py:259-341, ddbtsc.py:224-269,
-ddbtsci.py:240-297) runs as rocbta_dddbtasc_permuted /
-rocbta_dddbtasci_permuted.
+reference's "AX=B" branch is an empty placeholder, ddbtasc.py:183-185)
+raises NotImplementedError.  buffers= without rhs (ddbtasc.py:244-332,
+ddbtasci.py:259-341, ddbtsc.py:224-269, ddbtsci.py:240-297) runs as
+rocbta_dddbtasc_permuted / rocbta_dddbtasci_permuted, and with rhs= and
+quadratic=True (ddbtasc.py:478-710, ddbtasci.py:578-845, ddbtsc.py:357-466,
+ddbtsci.py:446-639) as rocbta_dddbtasc_quadratic_permuted /
+rocbta_dddbtasci_quadratic_permuted.
 """
 
 import torch
@@ -78,10 +80,10 @@ def _rhs_arrays(kwargs, what, arrow):
         if any(x is None for x in bufs):
             raise ValueError("buffers does not contain the correct arrays for A")
         if arrs is not None:
-            if any(buffers.get(k, None) is None
-                   for k in ("B_lower_buffer_blocks", "B_upper_buffer_blocks")):
+            bb = [buffers.get(k, None) for k in ("B_lower_buffer_blocks", "B_upper_buffer_blocks")]
+            if any(x is None for x in bb):
                 raise ValueError("buffers does not contain the correct arrays for B")
-            raise NotImplementedError(f"{what}: the permuted A X A^T = B form is not provided")
+            bufs += bb
     return arrs, bufs
 
 
@@ -107,12 +109,22 @@ def _run(fn_name, op, D, Lo, Up, Rl, Ru, T, n, b, a, dev, invert_last_block, dir
     if bufs is not None:  # permuted: the middle blocks 1 .. n-2
         if n < 2:
             raise IndexError(f"{what}: the permuted form needs at least 2 diagonal blocks, got {n}")
-        for x, nm in zip(bufs, ("A_lower_buffer_blocks", "A_upper_buffer_blocks")):
+        for x, nm in zip(bufs, ("A_lower_buffer_blocks", "A_upper_buffer_blocks",
+                                "B_lower_buffer_blocks", "B_upper_buffer_blocks")):
             _check_like(x, Lo, nm)
         ws, nbytes = workspace(op, n, b, a, 0, dev)
-        rc = getattr(lib, fn_name + "_permuted")(
-            n, b, a, ptr(D), ptr(Lo), ptr(Up), ptr(Rl), ptr(Ru), ptr(T), ptr(bufs[0]),
-            ptr(bufs[1]), ws.data_ptr(), nbytes, stream_of(dev))
+        if B is None:
+            rc = getattr(lib, fn_name + "_permuted")(
+                n, b, a, ptr(D), ptr(Lo), ptr(Up), ptr(Rl), ptr(Ru), ptr(T), ptr(bufs[0]),
+                ptr(bufs[1]), ws.data_ptr(), nbytes, stream_of(dev))
+        else:
+            for x, y, nm in zip(B, (D, Lo, Up, Rl, Ru, T), _B_KEYS):
+                _check_like(x, y, nm)
+            Bp = [ptr(x) for x in B] + [None] * (6 - len(B))
+            rc = getattr(lib, fn_name + "_quadratic_permuted")(
+                n, b, a, ptr(D), ptr(Lo), ptr(Up), ptr(Rl), ptr(Ru), ptr(T), ptr(bufs[0]),
+                ptr(bufs[1]), *Bp, ptr(bufs[2]), ptr(bufs[3]), ws.data_ptr(), nbytes,
+                stream_of(dev))
         _lib.check(rc, what)
         return
     ws, nbytes = workspace(op, n, b, a, 0, dev)

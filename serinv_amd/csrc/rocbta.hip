@@ -2621,6 +2621,242 @@ int rocbta_dddbtasci_permuted(int64_t n, int64_t b, int64_t a, double* D, double
   return ROCBTA_OK;
 }
 
+// Sums of more than two products per output: the terms go out in rounds of
+// two (a GEMM problem holds two), the first round with the output's beta
+// and the later ones accumulating.  Outputs of one list must not be read by
+// any term of the same list.
+namespace {
+struct Sums {
+  struct Sum {
+    double* C;
+    int64_t ldc, M, N;
+    double beta;
+    std::vector<GemmTerm> terms;
+  };
+  std::vector<Sum> v;
+  Sums& p(double* C, int64_t ldc, int64_t M, int64_t N, double beta) {
+    v.push_back({C, ldc, M, N, beta, {}});
+    return *this;
+  }
+  Sums& t(const double* A, int64_t lda, bool tA, const double* B, int64_t ldb, bool tB, int64_t k,
+          double alpha) {
+    if (k > 0) v.back().terms.push_back(term(A, lda, tA, B, ldb, tB, (int)k, alpha));
+    return *this;
+  }
+  int run(hipStream_t s) {
+    size_t rounds = 1;
+    for (const Sum& x : v) rounds = std::max(rounds, (x.terms.size() + 1) / 2);
+    for (size_t r = 0; r < rounds; ++r) {
+      GemmList g;
+      for (const Sum& x : v) {
+        if (x.M <= 0 || x.N <= 0 || 2 * r >= x.terms.size()) continue;
+        GemmProblem& pr = g.add(x.C, x.ldc, (int)x.M, (int)x.N, r == 0 ? x.beta : 1.0);
+        for (size_t k = 2 * r; k < std::min(2 * r + 2, x.terms.size()); ++k) push_term(pr, x.terms[k]);
+      }
+      int rc = run_gemms(g, s);
+      if (rc) return rc;
+    }
+    return ROCBTA_OK;
+  }
+};
+// A block and its leading dimension.
+struct Blk {
+  double* p;
+  int64_t ld;
+};
+}  // namespace
+
+int rocbta_dddbtasc_quadratic_permuted(int64_t n, int64_t b, int64_t a, double* D, double* Lo,
+                                       double* Up, double* Rl, double* Ru, double* T,
+                                       double* Lbuf, double* Ubuf, double* BD, double* BLo,
+                                       double* BUp, double* BRl, double* BRu, double* BT,
+                                       double* BLbuf, double* BUbuf, void* workspace,
+                                       size_t workspace_bytes, void* stream) {
+  int rc = dd_permuted_args(n, b, a, D, Lo, Up, Rl, Ru, T, Lbuf, Ubuf);
+  if (rc) return rc;
+  if (!BD || !BLo || !BUp || !BLbuf || !BUbuf || (a > 0 && (!BRl || !BRu || !BT)))
+    return ROCBTA_ERR_ARG;
+  const DdLayout lay = dd_layout(b, a);
+  if (!workspace || workspace_bytes < lay.total) return ROCBTA_ERR_WORKSPACE;
+  hipStream_t s = (hipStream_t)stream;
+  char* ws = (char*)workspace;
+  auto P = [&](size_t off) { return (double*)(ws + off); };
+  double *W = P(lay.w), *V = P(lay.v), *Y = P(lay.z), *Z = P(lay.q1);
+  const int64_t bb = b * b, ab = a * b;
+  // ddbtasc.py:498-501: the fill-ins start as A's and B's (0,1) and (1,0)
+  RB_CHECK(hipMemcpyAsync(Lbuf, Up, bb * sizeof(double), hipMemcpyDeviceToDevice, s));
+  RB_CHECK(hipMemcpyAsync(Ubuf, Lo, bb * sizeof(double), hipMemcpyDeviceToDevice, s));
+  RB_CHECK(hipMemcpyAsync(BLbuf, BUp, bb * sizeof(double), hipMemcpyDeviceToDevice, s));
+  RB_CHECK(hipMemcpyAsync(BUbuf, BLo, bb * sizeof(double), hipMemcpyDeviceToDevice, s));
+  for (int64_t i = 1; i + 1 < n; ++i) {  // ddbtasc.py:504-708
+    double* S = D + i * bb;
+    rc = block_inverse(S, b, (int)b, ws, lay, s);
+    if (rc) return rc;
+    // neighbours r = i+1, 0, t of block i: rows, F_r = A~_{r,i}, G_r =
+    // B~_{r,i}, H_r = B~_{i,r}, P_r = F_r S, M_r = F_r S B~_ii S^T
+    const int nr = a > 0 ? 3 : 2;
+    const int64_t rows[3] = {b, b, a};
+    const Blk F[3] = {{Lo + i * bb, b}, {Lbuf + (i - 1) * bb, b}, {Rl + i * ab, b}};
+    const Blk G[3] = {{BLo + i * bb, b}, {BLbuf + (i - 1) * bb, b}, {BRl + i * ab, b}};
+    const Blk H[3] = {{BUp + i * bb, b}, {BUbuf + (i - 1) * bb, b}, {BRu + i * ab, a}};
+    const Blk Pm[3] = {{P(lay.p1), b}, {P(lay.m1), b}, {P(lay.p2), b}};
+    const Blk Mm[3] = {{P(lay.q2), b}, {P(lay.q3), b}, {P(lay.m2), b}};
+    // B~'s output block (r, c)
+    const Blk Ob[3][3] = {{{BD + (i + 1) * bb, b}, {BUbuf + i * bb, b}, {BRu + (i + 1) * ab, a}},
+                          {{BLbuf + i * bb, b}, {BD, b}, {BRu, a}},
+                          {{BRl + (i + 1) * ab, b}, {BRl, b}, {BT, a}}};
+    // W = S A_{i,i+1}, Y = S A~_{i,0}, V = S A_{i,t}; P_r; Z = S B~_ii
+    Gq g1;
+    g1.p(W, b, b, b, 0.0).t(S, b, false, Up + i * bb, b, false, b, 1.0);
+    g1.p(Y, b, b, b, 0.0).t(S, b, false, Ubuf + (i - 1) * bb, b, false, b, 1.0);
+    if (a > 0) g1.p(V, a, b, a, 0.0).t(S, b, false, Ru + i * ab, a, false, b, 1.0);
+    for (int r = 0; r < nr; ++r)
+      g1.p(Pm[r].p, Pm[r].ld, rows[r], b, 0.0).t(F[r].p, F[r].ld, false, S, b, false, b, 1.0);
+    g1.p(Z, b, b, b, 0.0).t(S, b, false, BD + i * bb, b, false, b, 1.0);
+    if ((rc = g1.run(s))) return rc;
+    // A's updates (as rocbta_dddbtasc_permuted) and B~_ii <- Z S^T
+    const double* L = Lo + i * bb;
+    const double* R = Rl + i * ab;
+    const double* Lp = Lbuf + (i - 1) * bb;
+    Gq g2;
+    g2.p(D + (i + 1) * bb, b, b, b, 1.0).t(L, b, false, W, b, false, b, -1.0);
+    g2.p(Lbuf + i * bb, b, b, b, 0.0).t(Lp, b, false, W, b, false, b, -1.0);
+    g2.p(Ubuf + i * bb, b, b, b, 0.0).t(L, b, false, Y, b, false, b, -1.0);
+    g2.p(D, b, b, b, 1.0).t(Lp, b, false, Y, b, false, b, -1.0);
+    if (a > 0) {
+      g2.p(Rl + (i + 1) * ab, b, a, b, 1.0).t(R, b, false, W, b, false, b, -1.0);
+      g2.p(Ru + (i + 1) * ab, a, b, a, 1.0).t(L, b, false, V, a, false, b, -1.0);
+      g2.p(T, a, a, a, 1.0).t(R, b, false, V, a, false, b, -1.0);
+      g2.p(Rl, b, a, b, 1.0).t(R, b, false, Y, b, false, b, -1.0);
+      g2.p(Ru, a, b, a, 1.0).t(Lp, b, false, V, a, false, b, -1.0);
+    }
+    g2.p(BD + i * bb, b, b, b, 0.0).t(Z, b, false, S, b, true, b, 1.0);
+    if ((rc = g2.run(s))) return rc;
+    Gq g3;  // M_r = F_r B~_ii
+    for (int r = 0; r < nr; ++r)
+      g3.p(Mm[r].p, Mm[r].ld, rows[r], b, 0.0).t(F[r].p, F[r].ld, false, BD + i * bb, b, false, b, 1.0);
+    if ((rc = g3.run(s))) return rc;
+    // B~_rc += M_r F_c^T - G_r P_c^T - P_r H_c
+    Sums g4;
+    for (int r = 0; r < nr; ++r)
+      for (int c = 0; c < nr; ++c)
+        g4.p(Ob[r][c].p, Ob[r][c].ld, rows[r], rows[c], 1.0)
+            .t(Mm[r].p, Mm[r].ld, false, F[c].p, F[c].ld, true, b, 1.0)
+            .t(G[r].p, G[r].ld, false, Pm[c].p, Pm[c].ld, true, b, -1.0)
+            .t(Pm[r].p, Pm[r].ld, false, H[c].p, H[c].ld, false, b, -1.0);
+    if ((rc = g4.run(s))) return rc;
+  }
+  return ROCBTA_OK;
+}
+
+int rocbta_dddbtasci_quadratic_permuted(int64_t n, int64_t b, int64_t a, double* D, double* Lo,
+                                        double* Up, double* Rl, double* Ru, double* T,
+                                        double* Lbuf, double* Ubuf, double* BD, double* BLo,
+                                        double* BUp, double* BRl, double* BRu, double* BT,
+                                        double* BLbuf, double* BUbuf, void* workspace,
+                                        size_t workspace_bytes, void* stream) {
+  int rc = dd_permuted_args(n, b, a, D, Lo, Up, Rl, Ru, T, Lbuf, Ubuf);
+  if (rc) return rc;
+  if (!BD || !BLo || !BUp || !BLbuf || !BUbuf || (a > 0 && (!BRl || !BRu || !BT)))
+    return ROCBTA_ERR_ARG;
+  const DdLayout lay = dd_layout(b, a);
+  if (!workspace || workspace_bytes < lay.total) return ROCBTA_ERR_WORKSPACE;
+  hipStream_t s = (hipStream_t)stream;
+  char* ws = (char*)workspace;
+  auto P = [&](size_t off) { return (double*)(ws + off); };
+  double *E = P(lay.e), *Sm = P(lay.sm), *Mt = P(lay.mt), *Q1 = P(lay.q1), *Q2 = P(lay.q2);
+  double *Q3 = P(lay.q3), *Dc = P(lay.dc), *Bdc = P(lay.bdc), *Gm = P(lay.g);
+  const int64_t bb = b * b, ab = a * b;
+  const int nj = a > 0 ? 3 : 2;
+  for (int64_t i = n - 2; i >= 1; --i) {  // ddbtasci.py:618-839
+    // finished blocks j, k in (i+1, 0, t); sizes; A~'s row i (a_j) and
+    // column i (c_j); B~'s (bR_j, bC_j); X_jk and XB_jk
+    const int64_t sz[3] = {b, b, a};
+    const Blk Aa[3] = {{Up + i * bb, b}, {Ubuf + (i - 1) * bb, b}, {Ru + i * ab, a}};
+    const Blk Ac[3] = {{Lo + i * bb, b}, {Lbuf + (i - 1) * bb, b}, {Rl + i * ab, b}};
+    const Blk bR[3] = {{BUp + i * bb, b}, {BUbuf + (i - 1) * bb, b}, {BRu + i * ab, a}};
+    const Blk bC[3] = {{BLo + i * bb, b}, {BLbuf + (i - 1) * bb, b}, {BRl + i * ab, b}};
+    const Blk X[3][3] = {{{D + (i + 1) * bb, b}, {Ubuf + i * bb, b}, {Ru + (i + 1) * ab, a}},
+                         {{Lbuf + i * bb, b}, {D, b}, {Ru, a}},
+                         {{Rl + (i + 1) * ab, b}, {Rl, b}, {T, a}}};
+    const Blk XB[3][3] = {{{BD + (i + 1) * bb, b}, {BUbuf + i * bb, b}, {BRu + (i + 1) * ab, a}},
+                          {{BLbuf + i * bb, b}, {BD, b}, {BRu, a}},
+                          {{BRl + (i + 1) * ab, b}, {BRl, b}, {BT, a}}};
+    // B_j (b x sz_j), C_j (sz_j x b), DF_j (b x sz_j), DK_j (sz_j x b)
+    const Blk Bj[3] = {{P(lay.b1), b}, {P(lay.w), b}, {P(lay.b2), a}};
+    const Blk Cj[3] = {{P(lay.c1), b}, {P(lay.z), b}, {P(lay.c2), b}};
+    const Blk DF[3] = {{P(lay.df1), b}, {P(lay.p1), b}, {P(lay.df2), a}};
+    const Blk DK[3] = {{P(lay.dk1), b}, {P(lay.m1), b}, {P(lay.dk2), b}};
+    double* Di = D + i * bb;
+    double* Bi = BD + i * bb;
+    RB_CHECK(hipMemcpyAsync(Dc, Di, bb * sizeof(double), hipMemcpyDeviceToDevice, s));
+    RB_CHECK(hipMemcpyAsync(Bdc, Bi, bb * sizeof(double), hipMemcpyDeviceToDevice, s));
+    // B_j = sum_k a_k X_kj; C_j = sum_k X_jk c_k; DF_j = sum_k a_k XB_kj -
+    // sum_k bR_k X_jk^T; DK_j = sum_k X_jk bC_k - sum_k XB_jk a_k^T
+    Sums pa;
+    for (int j = 0; j < nj; ++j) {
+      pa.p(Bj[j].p, Bj[j].ld, b, sz[j], 0.0);
+      for (int k = 0; k < nj; ++k) pa.t(Aa[k].p, Aa[k].ld, false, X[k][j].p, X[k][j].ld, false, sz[k], 1.0);
+      pa.p(Cj[j].p, Cj[j].ld, sz[j], b, 0.0);
+      for (int k = 0; k < nj; ++k) pa.t(X[j][k].p, X[j][k].ld, false, Ac[k].p, Ac[k].ld, false, sz[k], 1.0);
+      pa.p(DF[j].p, DF[j].ld, b, sz[j], 0.0);
+      for (int k = 0; k < nj; ++k) {
+        pa.t(Aa[k].p, Aa[k].ld, false, XB[k][j].p, XB[k][j].ld, false, sz[k], 1.0);
+        pa.t(bR[k].p, bR[k].ld, false, X[j][k].p, X[j][k].ld, true, sz[k], -1.0);
+      }
+      pa.p(DK[j].p, DK[j].ld, sz[j], b, 0.0);
+      for (int k = 0; k < nj; ++k) {
+        pa.t(X[j][k].p, X[j][k].ld, false, bC[k].p, bC[k].ld, false, sz[k], 1.0);
+        pa.t(XB[j][k].p, XB[j][k].ld, false, Aa[k].p, Aa[k].ld, true, sz[k], -1.0);
+      }
+    }
+    if ((rc = pa.run(s))) return rc;
+    // E = sum_j B_j c_j; Sm = sum_j DF_j a_j^T - sum_j B_j bC_j; Mt = sum_j a_j C_j
+    Sums pb;
+    pb.p(E, b, b, b, 0.0);
+    for (int j = 0; j < nj; ++j) pb.t(Bj[j].p, Bj[j].ld, false, Ac[j].p, Ac[j].ld, false, sz[j], 1.0);
+    pb.p(Sm, b, b, b, 0.0);
+    for (int j = 0; j < nj; ++j) {
+      pb.t(DF[j].p, DF[j].ld, false, Aa[j].p, Aa[j].ld, true, sz[j], 1.0);
+      pb.t(Bj[j].p, Bj[j].ld, false, bC[j].p, bC[j].ld, false, sz[j], -1.0);
+    }
+    pb.p(Mt, b, b, b, 0.0);
+    for (int j = 0; j < nj; ++j) pb.t(Aa[j].p, Aa[j].ld, false, Cj[j].p, Cj[j].ld, false, sz[j], 1.0);
+    if ((rc = pb.run(s))) return rc;
+    // Q1 = S Sm, Q2 = S E, Q3 = B~'^T Mt^T, Gm = S E; the new row and column
+    // i of X B X^T (bR_j, bC_j) and of X (a_j, c_j)
+    Gq pc;
+    pc.p(Q1, b, b, b, 0.0).t(Dc, b, false, Sm, b, false, b, 1.0);
+    pc.p(Q2, b, b, b, 0.0).t(Dc, b, false, E, b, false, b, 1.0);
+    pc.p(Q3, b, b, b, 0.0).t(Bdc, b, true, Mt, b, true, b, 1.0);
+    for (int j = 0; j < nj; ++j) {
+      pc.p(bR[j].p, bR[j].ld, b, sz[j], 0.0)
+          .t(Dc, b, false, DF[j].p, DF[j].ld, false, b, -1.0)
+          .t(Bdc, b, false, Cj[j].p, Cj[j].ld, true, b, -1.0);
+      pc.p(bC[j].p, bC[j].ld, sz[j], b, 0.0)
+          .t(DK[j].p, DK[j].ld, false, Dc, b, true, b, 1.0)
+          .t(Cj[j].p, Cj[j].ld, false, Bdc, b, false, b, -1.0);
+      pc.p(Aa[j].p, Aa[j].ld, b, sz[j], 0.0).t(Dc, b, false, Bj[j].p, Bj[j].ld, false, b, -1.0);
+      pc.p(Ac[j].p, Ac[j].ld, sz[j], b, 0.0).t(Cj[j].p, Cj[j].ld, false, Dc, b, false, b, -1.0);
+    }
+    if ((rc = pc.run(s))) return rc;
+    Gq pd;  // X B X^T_ii = B~' + Q1 S^T + Q2 B~' (+ Q3 S^T below); X_ii = S + S E S
+    pd.p(Bi, b, b, b, 1.0).t(Q1, b, false, Dc, b, true, b, 1.0).t(Q2, b, false, Bdc, b, false, b, 1.0);
+    pd.p(Gm, b, b, b, 0.0).t(Dc, b, false, E, b, false, b, 1.0);
+    if ((rc = pd.run(s))) return rc;
+    Gq pe;
+    pe.p(Bi, b, b, b, 1.0).t(Q3, b, false, Dc, b, true, b, 1.0);
+    pe.p(Di, b, b, b, 1.0).t(Gm, b, false, Dc, b, false, b, 1.0);
+    if ((rc = pe.run(s))) return rc;
+  }
+  // ddbtasci.py:841-845
+  RB_CHECK(hipMemcpyAsync(Lo, Ubuf, bb * sizeof(double), hipMemcpyDeviceToDevice, s));
+  RB_CHECK(hipMemcpyAsync(Up, Lbuf, bb * sizeof(double), hipMemcpyDeviceToDevice, s));
+  RB_CHECK(hipMemcpyAsync(BLo, BUbuf, bb * sizeof(double), hipMemcpyDeviceToDevice, s));
+  RB_CHECK(hipMemcpyAsync(BUp, BLbuf, bb * sizeof(double), hipMemcpyDeviceToDevice, s));
+  return ROCBTA_OK;
+}
+
 // ------------------------------------------------------------- streaming
 // Host <-> device streaming forms: the block arrays stay in host memory
 // (pinned, for asynchronous copies), and two slots of device buffers per

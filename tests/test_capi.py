@@ -158,6 +158,18 @@ def test_ddbt_permuted_argument_errors(lib):
                                         fake, 8, None) == _lib.ROCBTA_ERR_WORKSPACE
 
 
+def test_ddbt_quadratic_permuted_argument_errors(lib):
+    fake = ctypes.c_void_p(16)
+    big = 1 << 24
+    # a missing B buffer, n < 2, then a workspace too small
+    assert lib.rocbta_dddbtasc_quadratic_permuted(3, 4, 2, *([fake] * 15), None, fake, big,
+                                                  None) == _lib.ROCBTA_ERR_ARG
+    assert lib.rocbta_dddbtasci_quadratic_permuted(1, 4, 2, *([fake] * 16), fake, big,
+                                                   None) == _lib.ROCBTA_ERR_ARG
+    assert lib.rocbta_dddbtasci_quadratic_permuted(3, 4, 2, *([fake] * 16), fake, 8,
+                                                   None) == _lib.ROCBTA_ERR_WORKSPACE
+
+
 def test_ddbt_quadratic_argument_errors(lib):
     fake = ctypes.c_void_p(16)
     big = 1 << 26

@@ -134,3 +134,89 @@ def test_ddbt_permuted_argument_errors(algs, gpu_device):
            g[4][:1].clone(), g[5].clone()]
     with pytest.raises(IndexError):
         algs.ddbtasc(*one, buffers={k: v[:0].clone() for k, v in bufs.items()})
+
+
+_BK = ("B_diagonal_blocks", "B_lower_diagonal_blocks", "B_upper_diagonal_blocks",
+       "B_lower_arrow_blocks", "B_upper_arrow_blocks", "B_arrow_tip_block")
+
+
+class GpuQRT:
+    """The A X A^T = B forms under the procedure's signatures."""
+
+    def __init__(self, algs, a):
+        self.A, self.bt = algs, a == 0
+
+    def _call(self, fa, fbt, arrs, brr, bufs=None):
+        kw = {"quadratic": True}
+        if bufs is not None:
+            kw["buffers"] = bufs
+        if self.bt:
+            fbt(*arrs[:3], rhs=dict(zip(_BK[:3], brr[:3])), **kw)
+        else:
+            fa(*arrs, rhs=dict(zip(_BK, brr)), **kw)
+
+    def sc(self, arrs, L, U, brr, BL, BU):
+        bufs = dict(bufdict(L, U), B_lower_buffer_blocks=BL, B_upper_buffer_blocks=BU)
+        self._call(self.A.ddbtasc, self.A.ddbtsc, arrs, brr, bufs)
+
+    def sci(self, arrs, L, U, brr, BL, BU):
+        bufs = dict(bufdict(L, U), B_lower_buffer_blocks=BL, B_upper_buffer_blocks=BU)
+        self._call(self.A.ddbtasci, self.A.ddbtsci, arrs, brr, bufs)
+
+    def rsc(self, r, rb):
+        self._call(self.A.ddbtasc, self.A.ddbtsc, r, rb)
+
+    def rsci(self, r, rb):
+        self._call(self.A.ddbtasci, self.A.ddbtsci, r, rb)
+
+
+class OracleQRT:
+    sc = staticmethod(lambda arrs, L, U, brr, BL, BU:
+                      serinv_np.ddbtasc_quadratic_permuted(*arrs, L, U, *brr, BL, BU))
+    sci = staticmethod(lambda arrs, L, U, brr, BL, BU:
+                       serinv_np.ddbtasci_quadratic_permuted(*arrs, L, U, *brr, BL, BU))
+    rsc = staticmethod(lambda r, rb: serinv_np.ddbtasc_quadratic(*r, *rb))
+    rsci = staticmethod(lambda r, rb: serinv_np.ddbtasci_quadratic(*r, *rb))
+
+
+def quad_case(b, a, n, seed):
+    rng = np.random.default_rng(seed)
+    A = generators.dd_bta(b, a, n, rng)
+    Bm = generators.dd_bta(b, a, n, rng)
+    generators.symmetrize(Bm)
+    return A, Bm
+
+
+@pytest.mark.parametrize("b,a,n", SHAPES)
+def test_ddbtasc_quadratic_permuted_matches_oracle(algs, gpu_device, b, a, n):
+    print(f"ddbtasc_quadratic_permuted b={b} a={a} n={n}", flush=True)
+    A, Bm = quad_case(b, a, n, 59000 + b + 7 * a + 13 * n)
+    arrs = list(dense.bta_dense_to_arrays_full(A, b, a, n))
+    brr = list(dense.bta_dense_to_arrays_full(Bm, b, a, n))
+    z = [np.zeros_like(arrs[1]) for _ in range(4)]
+    g = list(to_dev(gpu_device, *arrs, *brr, *z))
+    GpuQRT(algs, a).sc(g[:6], g[12], g[13], g[6:12], g[14], g[15])
+    serinv_np.ddbtasc_quadratic_permuted(*arrs, z[0], z[1], *brr, z[2], z[3])
+    for x, y in zip(to_host(*g), arrs + brr + z):
+        assert relerr(x, y) < TOL
+
+
+@pytest.mark.parametrize("b,a,n", SHAPES)
+def test_ddbtasci_quadratic_permuted_matches_oracle_and_dense(algs, gpu_device, b, a, n):
+    print(f"ddbtasci_quadratic_permuted b={b} a={a} n={n}", flush=True)
+    A, Bm = quad_case(b, a, n, 60000 + b + 7 * a + 13 * n)
+    arrs = list(dense.bta_dense_to_arrays_full(A, b, a, n))
+    brr = list(dense.bta_dense_to_arrays_full(Bm, b, a, n))
+    z = [np.zeros_like(arrs[1]) for _ in range(4)]
+    g = list(to_dev(gpu_device, *arrs, *brr, *z))
+    selinv_ddbt_permuted(g[:6], g[12], g[13], GpuQRT(algs, a), gpu_zeros(gpu_device), g[6:12],
+                         (g[14], g[15]))
+    selinv_ddbt_permuted(arrs, z[0], z[1], OracleQRT, np.zeros, brr, (z[2], z[3]))
+    h = to_host(*g)
+    for x, y in zip(h, arrs + brr + z):
+        assert relerr(x, y) < TOL
+    X = np.linalg.inv(A)
+    ref = list(dense.bta_dense_to_arrays_full(X, b, a, n)) + list(
+        dense.bta_dense_to_arrays_full(X @ Bm @ X.T, b, a, n))
+    for x, y in zip(h, ref):
+        assert relerr(x, y) < TOL

@@ -110,3 +110,89 @@ def test_ddbt_permuted_workload_n8_b32_a8():
     selinv_ddbt_permuted(arrs, Lbuf, Ubuf, OracleRT, np.zeros)
     for x, y in zip(arrs, dense.bta_dense_to_arrays_full(np.linalg.inv(A), b, a, n)):
         assert relerr(x, y) < 1e-11
+
+
+# --------------------------------------------------------------- A X A^T = B
+def transformed(A, B, elim, keep):
+    """B carried through the elimination of `elim`: M B M^T on `keep`, with
+    M = [-A_KE A_EE^{-1}, I] (what the quadratic sweep's updates B~_rc +=
+    F_r S B~_ii S^T F_c^T - B~_ri S^T F_c^T - F_r S B~_ic build up)."""
+    if not len(elim):
+        return B[np.ix_(keep, keep)]
+    M = np.hstack([-A[np.ix_(keep, elim)] @ np.linalg.inv(A[np.ix_(elim, elim)]), np.eye(len(keep))])
+    idx = np.concatenate([elim, keep])
+    return M @ B[np.ix_(idx, idx)] @ M.T
+
+
+class OracleQRT:
+    sc = staticmethod(lambda arrs, L, U, brr, BL, BU:
+                      serinv_np.ddbtasc_quadratic_permuted(*arrs, L, U, *brr, BL, BU))
+    sci = staticmethod(lambda arrs, L, U, brr, BL, BU:
+                       serinv_np.ddbtasci_quadratic_permuted(*arrs, L, U, *brr, BL, BU))
+    rsc = staticmethod(lambda r, rb: serinv_np.ddbtasc_quadratic(*r, *rb))
+    rsci = staticmethod(lambda r, rb: serinv_np.ddbtasci_quadratic(*r, *rb))
+
+
+def quad_case(b, a, n, seed):
+    rng = np.random.default_rng(seed)
+    A = generators.dd_bta(b, a, n, rng)
+    B = generators.dd_bta(b, a, n, rng)
+    generators.symmetrize(B)
+    return A, B
+
+
+@pytest.mark.parametrize("b,a,n", GRID)
+def test_ddbtasc_quadratic_permuted_blocks_vs_dense(b, a, n):
+    A, B = quad_case(b, a, n, 57000 + 100 * b + 10 * a + n)
+    arrs = list(dense.bta_dense_to_arrays_full(A, b, a, n))
+    brr = list(dense.bta_dense_to_arrays_full(B, b, a, n))
+    L, U = np.zeros_like(arrs[1]), np.zeros_like(arrs[1])
+    BL, BU = np.zeros_like(arrs[1]), np.zeros_like(arrs[1])
+    serinv_np.ddbtasc_quadratic_permuted(*arrs, L, U, *brr, BL, BU)
+    BD, BLo, BUp, BRl, BRu, BT = brr
+    Bref = dense.bta_dense_to_arrays_full(B, b, a, n)
+    assert np.array_equal(BLo, Bref[1]) and np.array_equal(BUp[1:], Bref[2][1:])
+    tip = np.arange(n * b, n * b + a)
+    for i in range(1, n - 1):  # middle block i after eliminating 1 .. i-1
+        elim = np.arange(b, i * b)
+        keep = np.concatenate([blk(i, b), tip])
+        Bt = transformed(A, B, elim, keep)
+        Sinv = np.linalg.inv(schur(A, elim, blk(i, b)))
+        assert relerr(BD[i], Sinv @ Bt[:b, :b] @ Sinv.T) < RTOL
+        assert relerr(BRl[i], Bt[b:, :b]) < RTOL
+        assert relerr(BRu[i], Bt[:b, b:]) < RTOL
+    for k in range(n - 1):  # B's fill-ins after eliminating 1 .. k
+        Bt = transformed(A, B, np.arange(b, (k + 1) * b), np.concatenate([blk(0, b), blk(k + 1, b)]))
+        assert relerr(BL[k], Bt[:b, b:]) < RTOL
+        assert relerr(BU[k], Bt[b:, :b]) < RTOL
+    keep = np.concatenate([blk(0, b), blk(n - 1, b), tip])
+    Bt = transformed(A, B, np.arange(b, (n - 1) * b), keep)
+    r0, r1, rt = slice(0, b), slice(b, 2 * b), slice(2 * b, 2 * b + a)
+    for x, y in ((BD[0], Bt[r0, r0]), (BD[n - 1], Bt[r1, r1]), (BT, Bt[rt, rt]), (BRl[0], Bt[rt, r0]),
+                 (BRl[n - 1], Bt[rt, r1]), (BRu[0], Bt[r0, rt]), (BRu[n - 1], Bt[r1, rt])):
+        assert relerr(x, y) < RTOL
+    # A's side is the plain permuted sweep
+    arrs2 = list(dense.bta_dense_to_arrays_full(A, b, a, n))
+    L2, U2 = np.zeros_like(L), np.zeros_like(U)
+    serinv_np.ddbtasc_permuted(*arrs2, L2, U2)
+    for x, y in zip(arrs + [L, U], arrs2 + [L2, U2]):
+        assert np.array_equal(x, y)
+
+
+@pytest.mark.parametrize("b,a,n", GRID)
+def test_ddbtasci_quadratic_permuted_vs_dense(b, a, n):
+    A, B = quad_case(b, a, n, 58000 + 100 * b + 10 * a + n)
+    X = np.linalg.inv(A)
+    XBX = X @ B @ X.T
+    arrs = list(dense.bta_dense_to_arrays_full(A, b, a, n))
+    brr = list(dense.bta_dense_to_arrays_full(B, b, a, n))
+    L, U = np.zeros_like(arrs[1]), np.zeros_like(arrs[1])
+    BL, BU = np.zeros_like(arrs[1]), np.zeros_like(arrs[1])
+    selinv_ddbt_permuted(arrs, L, U, OracleQRT, np.zeros, brr, (BL, BU))
+    for x, y in zip(arrs, dense.bta_dense_to_arrays_full(X, b, a, n)):
+        assert relerr(x, y) < RTOL
+    for x, y in zip(brr, dense.bta_dense_to_arrays_full(XBX, b, a, n)):
+        assert relerr(x, y) < RTOL
+    for k in range(n - 1):
+        assert relerr(BL[k], XBX[np.ix_(blk(0, b), blk(k + 1, b))]) < RTOL
+        assert relerr(BU[k], XBX[np.ix_(blk(k + 1, b), blk(0, b))]) < RTOL

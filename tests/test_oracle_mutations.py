@@ -309,3 +309,92 @@ DDBT_QSCI_MUTANTS = [(d, o, nw) for d in ("downward", "upward") for o, nw in _QS
 def test_ddbtsci_quadratic_mutants_are_caught(direction, old, new):
     bad = mutant(serinv_np.ddbtsci_quadratic, old, new)
     assert ddbt_quad_err(direction, sci=bad) > 1e-6
+
+
+# Non-SPD permuted forms (A X = I and A X A^T = B): mutants must change the
+# completed inverse (or A^{-1} B A^{-T}) against dense linear algebra.
+def ddbt_perm_err(sc=None, sci=None, quadratic=False):
+    from tests._ddbt_permuted import selinv_ddbt_permuted
+
+    b, a, n = 3, 2, 5
+    rng = np.random.default_rng(79)
+    A = generators.dd_bta(b, a, n, rng)
+    B = generators.dd_bta(b, a, n, rng)
+    generators.symmetrize(B)
+    arrs = list(dense.bta_dense_to_arrays_full(A, b, a, n))
+    L, U = np.zeros_like(arrs[1]), np.zeros_like(arrs[1])
+    X = np.linalg.inv(A)
+    if not quadratic:
+        class RT:
+            sc = staticmethod(lambda ar, l, u: (sc or serinv_np.ddbtasc_permuted)(*ar, l, u))
+            sci = staticmethod(lambda ar, l, u: (sci or serinv_np.ddbtasci_permuted)(*ar, l, u))
+            rsc = staticmethod(lambda r: serinv_np.ddbtasc(*r))
+            rsci = staticmethod(lambda r: serinv_np.ddbtasci(*r))
+        selinv_ddbt_permuted(arrs, L, U, RT, np.zeros)
+        got, ref = arrs, dense.bta_dense_to_arrays_full(X, b, a, n)
+    else:
+        brr = list(dense.bta_dense_to_arrays_full(B, b, a, n))
+        BL, BU = np.zeros_like(L), np.zeros_like(U)
+
+        class RT:
+            sc = staticmethod(lambda ar, l, u, br, bl, bu:
+                              (sc or serinv_np.ddbtasc_quadratic_permuted)(*ar, l, u, *br, bl, bu))
+            sci = staticmethod(lambda ar, l, u, br, bl, bu:
+                               (sci or serinv_np.ddbtasci_quadratic_permuted)(*ar, l, u, *br, bl, bu))
+            rsc = staticmethod(lambda r, rb: serinv_np.ddbtasc_quadratic(*r, *rb))
+            rsci = staticmethod(lambda r, rb: serinv_np.ddbtasci_quadratic(*r, *rb))
+        selinv_ddbt_permuted(arrs, L, U, RT, np.zeros, brr, (BL, BU))
+        got = arrs + brr
+        ref = list(dense.bta_dense_to_arrays_full(X, b, a, n)) + \
+            list(dense.bta_dense_to_arrays_full(X @ B @ X.T, b, a, n))
+    return max(np.max(np.abs(x - y)) / np.max(np.abs(y)) for x, y in zip(got, ref))
+
+
+@pytest.mark.parametrize("quadratic", [False, True])
+def test_unmutated_ddbt_permuted_oracle_passes(quadratic):
+    assert ddbt_perm_err(quadratic=quadratic) < TOL
+
+
+DDBT_PERM_MUTANTS = [
+    ("sc", "Ubuf[i] = -Lo[i] @ S @ Ubuf[i - 1]", "Ubuf[i] = -Up[i] @ S @ Ubuf[i - 1]"),
+    ("sc", "Ru[0] = Ru[0] - Lbuf[i - 1] @ S @ Ru[i]", "Ru[0] = Ru[0]"),
+    ("sci", "C2 = Lbuf[i] @ Lo[i] + D[0] @ Lbuf[i - 1] + Ru[0] @ Rl[i]",
+     "C2 = Lbuf[i] @ Lo[i] + D[0] @ Lbuf[i - 1]"),
+    ("sci", "B3 = Up[i] @ Ru[i + 1] + Ru[i] @ T + Ubuf[i - 1] @ Ru[0]",
+     "B3 = Up[i] @ Ru[i + 1] + Ru[i] @ T"),
+    ("sci", "        Lo[0] = Ubuf[0]  # X_{1,0}", "        Lo[0] = Lbuf[0]  # X_{1,0}"),
+]
+
+
+@pytest.mark.parametrize("which,old,new", DDBT_PERM_MUTANTS)
+def test_ddbt_permuted_mutants_are_caught(which, old, new):
+    if which == "sc":
+        err = ddbt_perm_err(sc=mutant(serinv_np.ddbtasc_permuted, old, new))
+    else:
+        err = ddbt_perm_err(sci=mutant(serinv_np.ddbtasci_permuted, old, new))
+    assert err > 1e-6
+
+
+DDBT_QPERM_MUTANTS = [
+    ("sc", "BUbuf[i] = BUbuf[i] + upd(\"n\", \"0\")", "BUbuf[i] = BUbuf[i] + upd(\"0\", \"n\")"),
+    ("sc", "return F[r] @ BD[i] @ F[c].T - G[r] @ S.T @ F[c].T - F[r] @ S @ H[c]",
+     "return F[r] @ BD[i] @ F[c].T - G[r] @ S.T @ F[c].T + F[r] @ S @ H[c]"),
+    ("sc", "BD[i] = S @ BD[i] @ S.T  # :581-583", "BD[i] = S @ BD[i]  # :581-583"),
+    ("sci", "Gj = [sum(bR[k] @ X[j][k].T for k in J) for j in J]",
+     "Gj = [sum(bR[k] @ X[j][k].T for k in range(2)) for j in J]"),
+    ("sci", "newC = [-DK[j] @ S.T - Cj[j] @ SB + Hj[j] @ S.T for j in J]",
+     "newC = [-DK[j] @ S.T - Cj[j] @ SB + Hj[j] @ S for j in J]"),
+    ("sci", "                 - S @ sum(Bj[j] @ bC[j] for j in J) @ S.T",
+     "                 - S @ sum(Bj[j] @ bC[j] for j in range(2)) @ S.T"),
+    ("sci", "        BLo[0] = BUbuf[0]", "        BLo[0] = BLbuf[0]"),
+]
+
+
+@pytest.mark.parametrize("which,old,new", DDBT_QPERM_MUTANTS)
+def test_ddbt_quadratic_permuted_mutants_are_caught(which, old, new):
+    if which == "sc":
+        err = ddbt_perm_err(sc=mutant(serinv_np.ddbtasc_quadratic_permuted, old, new), quadratic=True)
+    else:
+        err = ddbt_perm_err(sci=mutant(serinv_np.ddbtasci_quadratic_permuted, old, new),
+                            quadratic=True)
+    assert err > 1e-6
