@@ -376,7 +376,8 @@ def bench_streaming(dev, n, b, a, reps=2):
 
 
 def bench_ddbt(dev, n, b, a, reps=2):
-    """Non-SPD family: ddbtasc + ddbtasci on a diagonally dominant
+    """Non-SPD family: ddbtasc + ddbtasci (and the A X A^T = B and permuted
+    forms, one call each) on a diagonally dominant
     non-symmetric BTA matrix (random blocks in [0, 1), every diagonal entry
     raised by 3b + a + 1, generated on the device).  Property check: the
     diagonal blocks of A X are the identity, (A X)_ii = A_{i,i-1} X_{i-1,i}
@@ -413,8 +414,48 @@ def bench_ddbt(dev, n, b, a, reps=2):
         if i + 1 < n:
             ax = ax + Up0[i] @ Lo[i]
         err = max(err, float((ax - eye).abs().max()))
-    return {"n": n, "b": b, "a": a, "ddbtasc_ms": round(min(t_sc), 3),
-            "ddbtasci_ms": round(min(t_sci), 3), "max_abs_AX_ii_minus_I": float(f"{err:.3e}")}
+    out = {"n": n, "b": b, "a": a, "ddbtasc_ms": round(min(t_sc), 3),
+           "ddbtasci_ms": round(min(t_sci), 3), "max_abs_AX_ii_minus_I": float(f"{err:.3e}")}
+
+    # the A X A^T = B form (B symmetric) and the permuted (middle-partition)
+    # forms, one timed call each; the permuted pair is checked against the
+    # natural selected inverse on the middle blocks' rows after the reduced
+    # system (blocks 0, n-1, tip) is finished with the natural routines
+    def timed(fn):
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        fn()
+        e1.record(stream)
+        torch.cuda.synchronize()
+        return round(e0.elapsed_time(e1), 3)
+
+    Bs = [x.clone() for x in (D0, Lo0, Up0, Rl0, Ru0, T0)]
+    Bs[0] = 0.5 * (Bs[0] + Bs[0].transpose(1, 2))
+    Bs[2] = Bs[1].transpose(1, 2).contiguous()
+    Bs[4] = Bs[3].transpose(1, 2).contiguous()
+    Bs[5] = 0.5 * (Bs[5] + Bs[5].T)
+    keys = ("B_diagonal_blocks", "B_lower_diagonal_blocks", "B_upper_diagonal_blocks",
+            "B_lower_arrow_blocks", "B_upper_arrow_blocks", "B_arrow_tip_block")
+    A1 = [x.clone() for x in (D0, Lo0, Up0, Rl0, Ru0, T0)]
+    rhs = dict(zip(keys, Bs))
+    out["ddbtasc_quadratic_ms"] = timed(lambda: algs.ddbtasc(*A1, rhs=rhs, quadratic=True))
+    out["ddbtasci_quadratic_ms"] = timed(lambda: algs.ddbtasci(*A1, rhs=rhs, quadratic=True))
+    A2 = [x.clone() for x in (D0, Lo0, Up0, Rl0, Ru0, T0)]
+    bufs = {"A_lower_buffer_blocks": torch.zeros_like(Lo0), "A_upper_buffer_blocks": torch.zeros_like(Lo0)}
+    out["ddbtasc_permuted_ms"] = timed(lambda: algs.ddbtasc(*A2, buffers=bufs))
+    # reduced system: the natural routines on blocks 0, n-1 and the tip
+    red = [torch.stack([A2[0][0], A2[0][-1]]), bufs["A_upper_buffer_blocks"][-1:].clone(),
+           bufs["A_lower_buffer_blocks"][-1:].clone(), torch.stack([A2[3][0], A2[3][-1]]),
+           torch.stack([A2[4][0], A2[4][-1]]), A2[5].clone()]
+    algs.ddbtasc(*red)
+    algs.ddbtasci(*red)
+    A2[0][0], A2[0][-1] = red[0][0], red[0][1]
+    bufs["A_upper_buffer_blocks"][-1], bufs["A_lower_buffer_blocks"][-1] = red[1][0], red[2][0]
+    A2[3][0], A2[3][-1], A2[4][0], A2[4][-1] = red[3][0], red[3][1], red[4][0], red[4][1]
+    A2[5].copy_(red[5])
+    out["ddbtasci_permuted_ms"] = timed(lambda: algs.ddbtasci(*A2, buffers=bufs))
+    out["permuted_max_rel_diff_vs_natural"] = float(f"{float((A2[0] - D).abs().max() / D.abs().max()):.3e}")
+    return out
 
 
 def bench_mixed(dev, n, b, a, kappas=(1e4, 1e6, 1e8), r=16, reps=2):
