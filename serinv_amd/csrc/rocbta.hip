@@ -1070,48 +1070,48 @@ DdLayout dd_layout(int64_t b, int64_t a) {
 // A <- A^{-1} for a general m x m matrix (row stride ld), in place, by
 // blocked Gauss-Jordan elimination over 64-wide pivot blocks without
 // pivoting (the ddbt* blocks are diagonally dominant).  Step k, with
-// C = A[:, k] and R = P A[k, :] for P = A_kk^{-1} (tile kernel):
-//   A_ij -= C_i R_j (i, j != k);  A_ik = -C_i P (i != k);
-//   A_kj = R_j (j != k);  A_kk = P.
+// P = A_kk^{-1}:
+//   (1) A_kk <- P (tile kernel, in place);
+//   (2) A_kj <- P A_kj (j != k);
+//   (3) A_ij -= A_ik A_kj (i, j != k; A_kj is the new row block);
+//   (4) A_ik <- -A_ik P (i != k).
+// (2) and (4) are in place: the workgroup of an output tile reads, of the
+// aliased operand, only that tile (K = the pivot block's width), and reads
+// it in full before its epilogue writes.
 int block_inverse(double* A, int64_t ld, int m, char* ws, const DdLayout& lay, hipStream_t s) {
-  double* P = (double*)(ws + lay.p);
-  double* R = (double*)(ws + lay.r);
-  double* C = (double*)(ws + lay.c);
+  (void)ws;
+  (void)lay;
   for (int k0 = 0; k0 < m; k0 += TT) {
     const int tk = std::min(TT, m - k0), k1 = k0 + tk;
-    tile_getri_kernel<<<1, 256, 0, s>>>(A + (int64_t)k0 * ld + k0, ld, tk, P);
+    double* Pk = A + (int64_t)k0 * ld + k0;
+    tile_getri_kernel<<<1, 256, 0, s>>>(Pk, ld, tk, Pk, ld);
     RB_CHECK(hipGetLastError());
-    if (tk < m) {
-      // C = A[:, k0:k1] (m x tk, row stride TT); R = P A[k0:k1, :] (tk x m)
-      RB_CHECK(hipMemcpy2DAsync(C, TT * sizeof(double), A + k0, ld * sizeof(double),
-                                tk * sizeof(double), m, hipMemcpyDeviceToDevice, s));
-      GemmList gr;
-      push_term(gr.add(R, m, tk, m, 0.0), term(P, TT, false, A + (int64_t)k0 * ld, ld, false, tk, 1.0));
-      int rc = run_gemms(gr, s);
-      if (rc) return rc;
-      GemmList up;
-      const int rows[2][2] = {{0, k0}, {k1, m}};
-      for (const auto& rr : rows) {
-        const int r0 = rr[0], nr = rr[1] - rr[0];
-        if (nr <= 0) continue;
-        const double* Ci = C + (int64_t)r0 * TT;
-        if (k0 > 0)
-          push_term(up.add(A + (int64_t)r0 * ld, ld, nr, k0, 1.0),
-                    term(Ci, TT, false, R, m, false, tk, -1.0));
-        if (k1 < m)
-          push_term(up.add(A + (int64_t)r0 * ld + k1, ld, nr, m - k1, 1.0),
-                    term(Ci, TT, false, R + k1, m, false, tk, -1.0));
-        push_term(up.add(A + (int64_t)r0 * ld + k0, ld, nr, tk, 0.0),
-                  term(Ci, TT, false, P, TT, false, tk, -1.0));
-      }
-      rc = run_gemms(up, s);
-      if (rc) return rc;
-      RB_CHECK(hipMemcpy2DAsync(A + (int64_t)k0 * ld, ld * sizeof(double), R, m * sizeof(double),
-                                m * sizeof(double), tk, hipMemcpyDeviceToDevice, s));
+    if (tk == m) continue;
+    double* Rk = A + (int64_t)k0 * ld;  // row block k
+    const int cols[2][2] = {{0, k0}, {k1, m}};
+    GemmList g2;
+    for (const auto& cc : cols) {
+      const int c0 = cc[0], nc = cc[1] - cc[0];
+      if (nc > 0)
+        push_term(g2.add(Rk + c0, ld, tk, nc, 0.0), term(Pk, ld, false, Rk + c0, ld, false, tk, 1.0));
     }
-    RB_CHECK(hipMemcpy2DAsync(A + (int64_t)k0 * ld + k0, ld * sizeof(double), P,
-                              TT * sizeof(double), tk * sizeof(double), tk,
-                              hipMemcpyDeviceToDevice, s));
+    int rc = run_gemms(g2, s);
+    if (rc) return rc;
+    GemmList g3, g4;
+    for (const auto& rr : cols) {
+      const int r0 = rr[0], nr = rr[1] - rr[0];
+      if (nr <= 0) continue;
+      double* Cik = A + (int64_t)r0 * ld + k0;  // A_ik, rows r0 ..
+      for (const auto& cc : cols) {
+        const int c0 = cc[0], nc = cc[1] - cc[0];
+        if (nc > 0)
+          push_term(g3.add(A + (int64_t)r0 * ld + c0, ld, nr, nc, 1.0),
+                    term(Cik, ld, false, Rk + c0, ld, false, tk, -1.0));
+      }
+      push_term(g4.add(Cik, ld, nr, tk, 0.0), term(Cik, ld, false, Pk, ld, false, tk, -1.0));
+    }
+    if ((rc = run_gemms(g3, s))) return rc;
+    if ((rc = run_gemms(g4, s))) return rc;
   }
   return ROCBTA_OK;
 }
