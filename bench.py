@@ -270,6 +270,44 @@ def bench_batched(dev, nmat=64, r=4, reps=3, tune=True, mixed=False):
     }
 
 
+def bench_modes(dev, n, b, a, a_wide=96):
+    """The non-default pobtaf modes, one timed call each on n blocks:
+    factorize_last_block=False (the distributed root's call, ppobtaf.py:
+    110-118; on the chain kernel, with the last block and tip rebuilt) next to
+    the default, and an arrow wider than one 64-tile (a_wide), which takes
+    the launch-per-tile path.  The factorize_last_block=False result is
+    checked against the default run's factor on the blocks both define."""
+    import torch
+
+    import serinv_amd.algs as algs
+    from serinv_amd.utils.matrices import bta_spd
+
+    stream = torch.cuda.current_stream(dev)
+
+    def timed(fn):
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        fn()
+        e1.record(stream)
+        torch.cuda.synchronize()
+        return round(e0.elapsed_time(e1), 3)
+
+    dev_of = lambda x: None if x is None else torch.from_numpy(x).to(dev)  # noqa: E731
+    D0, L0, R0, T0, _ = (dev_of(x) for x in bta_spd(n, b, a, seed=63))
+    res = {"n": n, "b": b, "a": a}
+    D, L, R, T = (x.clone() for x in (D0, L0, R0, T0))
+    res["pobtaf_ms"] = timed(lambda: algs.pobtaf(D, L, R, T))
+    Do, Lo, Ro, To = (x.clone() for x in (D0, L0, R0, T0))
+    res["pobtaf_open_last_ms"] = timed(lambda: algs.pobtaf(Do, Lo, Ro, To, factorize_last_block=False))
+    diff = max(float((x[:-1] - y[:-1]).abs().max() / y[:-1].abs().max()) for x, y in ((Do, D), (Ro, R)))
+    res["open_last_max_rel_diff_blocks_0_to_n-2"] = float(f"{diff:.3e}")
+    del D, L, R, T, Do, Lo, Ro, To
+    Dw, Lw, Rw, Tw, _ = (dev_of(x) for x in bta_spd(n, b, a_wide, seed=64))
+    res["a_wide"] = a_wide
+    res["pobtaf_a_wide_ms"] = timed(lambda: algs.pobtaf(Dw, Lw, Rw, Tw))
+    return res
+
+
 def bench_permuted(dev, n, b, a, reps=2):
     """One partition's work in the distributed form: the permuted ("middle
     partition") pobtaf and pobtasi on n blocks, completed through the 2-block
@@ -704,6 +742,9 @@ def main():
                   f"err X {max(m['err_X'].values()):.2e}, refined x {m['err_x_refined']:.2e}",
                   file=sys.stderr, flush=True)
     if not args.no_permuted:
+        print("[bench] non-default pobtaf modes, n=64 ...", file=sys.stderr, flush=True)
+        out["modes"] = bench_modes(dev, 64, b, a)
+        print(f"[bench] modes {out['modes']}", file=sys.stderr, flush=True)
         print("[bench] permuted (middle-partition) form, n=64 ...", file=sys.stderr, flush=True)
         out["permuted"] = bench_permuted(dev, 64, b, a)
         print(f"[bench] permuted {out['permuted']}", file=sys.stderr, flush=True)

@@ -510,7 +510,7 @@ SiLayout si_layout(int64_t n, int64_t b, int64_t a) {
 }
 
 struct AfLayout {
-  size_t tinv, counter, claim, chain_cu, hflags, cflags, linv, tipp, total;
+  size_t tinv, counter, claim, chain_cu, hflags, cflags, linv, tipp, save_d, save_r, save_t, total;
   int nt, NT, slots, nstages, ntasks;
 };
 
@@ -535,6 +535,11 @@ AfLayout af_layout(int64_t n, int64_t b, int64_t a) {
   l.cflags = take((size_t)2 * l.NT * sizeof(int));
   l.linv = take((size_t)l.NT * CT * CT * sizeof(double));
   l.tipp = take((size_t)n * a * a * sizeof(double));
+  // factorize_last_block = 0 on the chain: the last block, its arrow block
+  // and the tip as given
+  l.save_d = take((size_t)b * b * sizeof(double));
+  l.save_r = take((size_t)a * b * sizeof(double));
+  l.save_t = take((size_t)a * a * sizeof(double));
   l.total = off;
   return l;
 }
@@ -554,10 +559,31 @@ int device_cus() {
 
 // ROCBTA_POBTAF_ALGO=loop forces the launch-per-tile path (used by tests to
 // cover both paths); default: the persistent chain kernel whenever it applies.
-bool pobtaf_use_chain(int64_t a, int factorize_last_block) {
+bool pobtaf_use_chain(int64_t a) {
   const char* e = getenv("ROCBTA_POBTAF_ALGO");
   if (e && strcmp(e, "loop") == 0) return false;
-  return factorize_last_block && a <= CT;
+  return a <= CT;
+}
+
+// info = 0 where it names a row of the last block or of the tip (rows past
+// `first_open`), which factorize_last_block = 0 leaves unfactored.
+__global__ void clear_info_above_kernel(int* info, int64_t first_open) {
+  if (threadIdx.x == 0 && *info > first_open) *info = 0;
+}
+
+// T = T0 - sum_{i < nblk} tipp[i], summed in block order (the order the
+// chain's tip task uses).  The chain forms only the lower part of each
+// symmetric tipp[i] (its upper-right 32 x 32 quadrant stays zero), so both
+// (r, s) and (s, r) take the lower entry.
+__global__ void tip_open_kernel(double* T, const double* T0, const double* tipp, int a, int nblk) {
+  const int aa = a * a;
+  for (int idx = blockIdx.x * blockDim.x + threadIdx.x; idx < aa; idx += gridDim.x * blockDim.x) {
+    const int r = idx / a, c = idx % a;
+    const int lo = r >= c ? idx : c * a + r;
+    double t = T0[idx];
+    for (int i = 0; i < nblk; ++i) t -= tipp[(int64_t)i * aa + lo];
+    T[idx] = t;
+  }
 }
 
 PobtafChain chain_params(int64_t n, int64_t b, int64_t a, double* D, double* L, double* R,
@@ -1192,10 +1218,47 @@ int rocbta_dpobtaf(int64_t n, int64_t b, int64_t a, double* D, double* L, double
   if (!info) return ROCBTA_ERR_ARG;
   set_int_kernel<<<1, 64, 0, s>>>(info, 0);
   RB_CHECK(hipGetLastError());
-  if (pobtaf_use_chain(a, factorize_last_block))
-    return pobtaf_chain(n, b, a, D, L, R, T, info, (char*)workspace, lay, s);
   const int bi = (int)b, ai = (int)a;
   const int64_t bb = b * b, ab = a * b;
+  if (!factorize_last_block && n == 1) return ROCBTA_OK;  // no block row to eliminate
+  if (pobtaf_use_chain(a)) {
+    char* ws = (char*)workspace;
+    if (factorize_last_block) return pobtaf_chain(n, b, a, D, L, R, T, info, ws, lay, s);
+    // pobtaf.py:143-159 leaves D_{n-1}, R_{n-1} and T Schur-updated but not
+    // factored.  The chain factors them too; they are rebuilt afterwards
+    // from the values given: D_{n-1} - L_{n-1,n-2} L_{n-1,n-2}^T, R_{n-1} -
+    // L_{t,n-2} L_{n-1,n-2}^T and T - sum_{i<n-1} L_{t,i} L_{t,i}^T (the
+    // chain's per-block tip products).  Pivots of those blocks do not count.
+    double* Dl = D + (n - 1) * bb;
+    double* Rl = R + (n - 1) * ab;
+    double* sd = (double*)(ws + lay.save_d);
+    double* sr = (double*)(ws + lay.save_r);
+    double* st = (double*)(ws + lay.save_t);
+    RB_CHECK(hipMemcpyAsync(sd, Dl, bb * sizeof(double), hipMemcpyDeviceToDevice, s));
+    if (a > 0) {
+      RB_CHECK(hipMemcpyAsync(sr, Rl, ab * sizeof(double), hipMemcpyDeviceToDevice, s));
+      RB_CHECK(hipMemcpyAsync(st, T, a * a * sizeof(double), hipMemcpyDeviceToDevice, s));
+    }
+    int rc = pobtaf_chain(n, b, a, D, L, R, T, info, ws, lay, s);
+    if (rc) return rc;
+    clear_info_above_kernel<<<1, 64, 0, s>>>(info, (n - 1) * b);
+    RB_CHECK(hipGetLastError());
+    RB_CHECK(hipMemcpyAsync(Dl, sd, bb * sizeof(double), hipMemcpyDeviceToDevice, s));
+    if (a > 0) RB_CHECK(hipMemcpyAsync(Rl, sr, ab * sizeof(double), hipMemcpyDeviceToDevice, s));
+    const double* Lp = L + (n - 2) * bb;
+    GemmList upd;
+    push_term(upd.add(Dl, b, bi, bi, 1.0), term(Lp, b, false, Lp, b, true, bi, -1.0));
+    if (a > 0)
+      push_term(upd.add(Rl, b, ai, bi, 1.0), term(R + (n - 2) * ab, b, false, Lp, b, true, bi, -1.0));
+    rc = run_gemms(upd, s);
+    if (rc) return rc;
+    if (a > 0) {
+      tip_open_kernel<<<cdiv(a * a, 256), 256, 0, s>>>(T, st, (double*)(ws + lay.tipp), ai,
+                                                      (int)(n - 1));
+      RB_CHECK(hipGetLastError());
+    }
+    return ROCBTA_OK;
+  }
   for (int64_t i = 0; i + 1 < n; ++i) {
     double* Di = D + i * bb;
     std::vector<Panel> panels = {{L + i * bb, bi, b}};

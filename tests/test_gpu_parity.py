@@ -182,6 +182,31 @@ def test_partial_flags(algs, gpu_device, monkeypatch, solve_algo, n, b, a):
         assert relerr(to_host(bb)[0], Bo) < TOL
 
 
+@pytest.mark.parametrize("algo", ["chain", "loop"])
+@pytest.mark.parametrize("n,b,a", [(1, 8, 3), (2, 64, 0), (3, 70, 5), (5, 130, 64), (4, 256, 32)])
+def test_factorize_last_block_false(algs, gpu_device, monkeypatch, algo, n, b, a):
+    """pobtaf.py:143-159 without :161-183: the chain path factors the last
+    block and the tip too and rebuilds them from the values given; the
+    launch-per-tile path never factors them.  Both against the oracle, and
+    with a tip that is not positive definite (zero, as the reference's
+    distributed wrappers hand to their inner partitions), which must raise
+    no error since it is not factored."""
+    if algo == "loop":
+        monkeypatch.setenv("ROCBTA_POBTAF_ALGO", "loop")
+    else:
+        monkeypatch.delenv("ROCBTA_POBTAF_ALGO", raising=False)
+    print(f"factorize_last_block=False {algo} n={n} b={b} a={a}", flush=True)
+    D, L, R, T, _ = bta_spd(n, b, a, seed=950 + n + b, n_rhs=1)
+    for zero_tip in (False, True):
+        T0 = np.zeros_like(T) if zero_tip else T
+        Do, Lo, Ro, To = (x.copy() for x in (D, L, R, T0))
+        serinv_np.pobtaf(Do, Lo, Ro, To, factorize_last_block=False)
+        d, l, r, t = to_dev(gpu_device, D, L, R, T0)
+        algs.pobtaf(d, l, r, t, factorize_last_block=False)
+        for g, o in zip(to_host(d, l, r, t), (Do, Lo, Ro, To)):
+            assert relerr(g, o) < TOL
+
+
 def test_vector_rhs(algs, gpu_device):
     D, L, R, T, B = bta_spd(3, 10, 4, seed=3, n_rhs=1)
     Do, Lo, Ro, To = (x.copy() for x in (D, L, R, T))
