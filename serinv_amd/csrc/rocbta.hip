@@ -19,6 +19,7 @@
 #include "solve_f64.hpp"
 #include "solve_sweep.hpp"
 #include "solve_block.hpp"
+#include "solve_wide.hpp"
 #include "resid_f64.hpp"
 #include "tile_f64.hpp"
 #include "getri_f64.hpp"
@@ -725,11 +726,12 @@ __global__ void fill_u64_kernel(unsigned long long* p, int64_t count, unsigned l
 
 // Solve algorithm: the block-level sweep (precomputed diagonal-block
 // inverses, two dependent device-wide products per block row) for large
-// blocks, the 64-row-tile sweep otherwise, the launch-per-tile form for
-// partial solves on small blocks.  ROCBTA_POBTAS_ALGO = block | sweep | loop
-// forces one (tests cover all three; block falls back when the shape does
-// not fit it).
-enum SolveAlgo { SOLVE_LOOP = 0, SOLVE_SWEEP = 1, SOLVE_BLOCK = 2 };
+// blocks and at most 8 right-hand sides, its wide form (64 columns per
+// launch, MFMA tiles) for more, the 64-row-tile sweep otherwise, the
+// launch-per-tile form for partial solves on small blocks.
+// ROCBTA_POBTAS_ALGO = block | wide | sweep | loop forces one (tests cover
+// all four; block and wide fall back when the shape does not fit them).
+enum SolveAlgo { SOLVE_LOOP = 0, SOLVE_SWEEP = 1, SOLVE_BLOCK = 2, SOLVE_WIDE = 3 };
 constexpr int BS_MIN_B = 256;   // default: block sweep from this block size on
 constexpr int BS_MAX_G = 1024;  // workgroups of the block sweep (tip partial sums sized for it)
 
@@ -741,12 +743,18 @@ bool bs_fits(int64_t a, int64_t r) { return a * bs_cw(r) <= BS_TIPMAX; }
 // sweep).
 constexpr int BS_MAX_R = 8;
 
+// The wide sweep: full solves with b a multiple of its 16-row tiles.
+bool wd_fits(int64_t b, int partial) { return !partial && b >= WD_CW && b % WD_T == 0; }
+
 int pobtas_algo(int64_t b, int64_t a, int64_t r, int partial) {
   const char* e = getenv("ROCBTA_POBTAS_ALGO");
   if (e && strcmp(e, "loop") == 0) return SOLVE_LOOP;
+  if (e && strcmp(e, "wide") == 0)
+    return wd_fits(b, partial) ? SOLVE_WIDE : (partial ? SOLVE_LOOP : SOLVE_SWEEP);
   if (e && strcmp(e, "sweep") == 0) return partial ? SOLVE_LOOP : SOLVE_SWEEP;
   if (e && strcmp(e, "block") == 0) return bs_fits(a, r) ? SOLVE_BLOCK : (partial ? SOLVE_LOOP : SOLVE_SWEEP);
   if (b >= BS_MIN_B && r <= BS_MAX_R && bs_fits(a, r)) return SOLVE_BLOCK;
+  if (b >= BS_MIN_B && r > BS_MAX_R && wd_fits(b, partial)) return SOLVE_WIDE;
   return partial ? SOLVE_LOOP : SOLVE_SWEEP;
 }
 
@@ -965,6 +973,133 @@ int pobtas_block(int64_t n, int64_t b, int64_t a, const double* D, const double*
   return ROCBTA_OK;
 }
 
+// Workspace of the wide sweep: the status word first (read by the caller),
+// the block inverses and transposes as for the block sweep, the published
+// rows of one 64-column launch, and the tip's per-block products.
+struct WdLayout {
+  size_t status, linv, lt, w, tinv, wt, sbuf, tbuf, tipp, tipb, total;
+};
+
+WdLayout wd_layout(int64_t n, int64_t b, int64_t a) {
+  WdLayout l{};
+  size_t off = 0;
+  auto take = [&](size_t bytes) {
+    size_t o = off;
+    off = align256(off + bytes);
+    return o;
+  };
+  const int64_t ntb = (b + TT - 1) / TT, nta = (a + TT - 1) / TT;
+  l.status = take(sizeof(int));
+  l.linv = take((size_t)n * b * b * sizeof(double));
+  l.lt = take((size_t)std::max<int64_t>(n - 1, 1) * b * b * sizeof(double));
+  l.w = take((size_t)n * std::max<int64_t>(ntb * TT * TT, ntb * TT * ntb * TT / 4) * sizeof(double));
+  l.tinv = take((size_t)a * a * sizeof(double));
+  l.wt = take((size_t)nta * TT * TT * sizeof(double));
+  l.sbuf = take((size_t)n * b * WD_CW * sizeof(double));
+  l.tbuf = take((size_t)n * b * WD_CW * sizeof(double));
+  l.tipp = take((size_t)n * a * WD_CW * sizeof(double));
+  l.tipb = take((size_t)a * WD_CW * sizeof(double));
+  l.total = off;
+  return l;
+}
+
+// pobtas with the wide sweep, per group of at most 64 right-hand-side
+// columns.  Forward: the sweep gives y_0 .. y_{n-1}; then the tip, y_t =
+// Tinv (b_t - sum_i R_i y_i) (per-block products, a fixed-order sum, one
+// product).  Backward: x_t = Tinv^T y_t and y'_i = y_i - R_i^T x_t (one
+// batched product) first, then the sweep.
+int pobtas_wide(int64_t n, int64_t b, int64_t a, const double* D, const double* L,
+                const double* R, const double* T, double* B, int64_t r, int64_t ldb, int trans,
+                char* ws, hipStream_t s) {
+  const WdLayout lay = wd_layout(n, b, a);
+  int* status = (int*)(ws + lay.status);
+  set_int_kernel<<<1, 64, 0, s>>>(status, 0);
+  RB_CHECK(hipGetLastError());
+  double* Linv = (double*)(ws + lay.linv);
+  double* Tinv = (double*)(ws + lay.tinv);
+  double* tipp = (double*)(ws + lay.tipp);
+  double* tipb = (double*)(ws + lay.tipb);
+  int rc = block_trtri(D, b, b * b, Linv, b, b * b, (int)b, (int)n, (double*)(ws + lay.w), s, true);
+  if (rc) return rc;
+  if (a > 0) {
+    rc = block_trtri(T, a, 0, Tinv, a, a * a, (int)a, 1, (double*)(ws + lay.wt), s);
+    if (rc) return rc;
+  }
+  if (trans) {  // the backward sweep reads L_{i+1,i}^T and Linv_i^T by rows
+    const int Tt = (int)((b + 63) / 64);
+    for (int64_t z0 = 0; z0 < n; z0 += 65535) {
+      const int nz = (int)std::min<int64_t>(65535, n - z0);
+      transpose_blocks_inplace_kernel<<<dim3(Tt * (Tt + 1) / 2, nz), 256, 0, s>>>(
+          Linv + z0 * b * b, (int)b, b * b);
+      RB_CHECK(hipGetLastError());
+    }
+    for (int64_t z0 = 0; z0 + 1 < n; z0 += 65535) {
+      const int nz = (int)std::min<int64_t>(65535, n - 1 - z0);
+      transpose_blocks_kernel<<<dim3(Tt * Tt, nz), 256, 0, s>>>(
+          L + z0 * b * b, (double*)(ws + lay.lt) + z0 * b * b, (int)b, b * b, Tt);
+      RB_CHECK(hipGetLastError());
+    }
+  }
+  WideSweepP P{};
+  P.L = trans ? (const double*)(ws + lay.lt) : L;
+  P.Linv = Linv;
+  P.ldb = ldb;
+  P.n = (int)n;
+  P.b = (int)b;
+  P.trans = trans;
+  P.sbuf = (double*)(ws + lay.sbuf);
+  P.tbuf = (double*)(ws + lay.tbuf);
+  P.status = status;
+  P.spin_limit = 1ll << 22;
+  const int64_t nempty = (int64_t)n * b * WD_CW * 2;  // sbuf and tbuf (adjacent)
+  const int ai = (int)a, bi = (int)b;
+  const int64_t ab = a * b;
+  double* Bt = B + n * b * ldb;  // tip rows
+  for (int64_t c0 = 0; c0 < r; c0 += WD_CW) {
+    const int rc_ = (int)std::min<int64_t>(WD_CW, r - c0);
+    if (trans && a > 0) {
+      // x_t = Tinv^T y_t, then y'_i = y_i - R_i^T x_t for every block
+      wide_copy_rows_kernel<<<cdiv(ai * rc_, 256), 256, 0, s>>>(tipb, rc_, Bt + c0, ldb, ai, rc_);
+      RB_CHECK(hipGetLastError());
+      GemmList xt;
+      push_term(xt.add(Bt + c0, ldb, ai, rc_, 0.0),
+                term(Tinv, a, true, tipb, rc_, false, ai, 1.0, TRI_UPPER, TRI_NONE));
+      if ((rc = run_gemms(xt, s))) return rc;
+      GemmList ry;
+      push_term(ry.add(B + c0, ldb, bi, rc_, 1.0, TRI_NONE, (int)n, b * ldb),
+                term(R, b, true, Bt + c0, ldb, false, ai, -1.0, TRI_NONE, TRI_NONE, ab, 0));
+      if ((rc = run_gemms(ry, s))) return rc;
+    }
+    P.B = B + c0;
+    P.r = rc_;
+    fill_u64_kernel<<<(int)std::min<int64_t>(4096, (nempty + 255) / 256), 256, 0, s>>>(
+        (unsigned long long*)P.sbuf, nempty, SW_EMPTY);
+    RB_CHECK(hipGetLastError());
+    const int nitems = (bi / WD_T) * ((rc_ + WD_T - 1) / WD_T);
+    P.G = std::min(std::min(nitems, WD_MAX_G), device_cus());
+    if (trans)
+      pobtas_wide_kernel<1><<<P.G, WD_THREADS, 0, s>>>(P);
+    else
+      pobtas_wide_kernel<0><<<P.G, WD_THREADS, 0, s>>>(P);
+    RB_CHECK(hipGetLastError());
+    if (!trans && a > 0) {
+      // y_t = Tinv (b_t - sum_i R_i y_i)
+      GemmList pr;
+      push_term(pr.add(tipp, rc_, ai, rc_, 0.0, TRI_NONE, (int)n, ai * rc_),
+                term(R, b, false, B + c0, ldb, false, bi, 1.0, TRI_NONE, TRI_NONE, ab, b * ldb));
+      if ((rc = run_gemms(pr, s))) return rc;
+      wide_tip_reduce_kernel<<<cdiv(ai * rc_, 256), 256, 0, s>>>(tipb, Bt + c0, ldb, tipp, ai, rc_,
+                                                               (int)n);
+      RB_CHECK(hipGetLastError());
+      GemmList yt;
+      push_term(yt.add(Bt + c0, ldb, ai, rc_, 0.0),
+                term(Tinv, a, false, tipb, rc_, false, ai, 1.0, TRI_LOWER, TRI_NONE));
+      if ((rc = run_gemms(yt, s))) return rc;
+    }
+  }
+  return ROCBTA_OK;
+}
+
 // Workspace of the permuted selected inversion: the inverses of the diagonal
 // blocks (all n, by the batched trtri), its scratch, copies of the factor
 // blocks a step overwrites while still reading them (L_{i+1,i}, L_{top,i},
@@ -1179,6 +1314,7 @@ size_t rocbta_workspace_bytes(int op, int64_t n, int64_t b, int64_t a, int64_t r
       size_t need = sw_layout(n, b, a, rr).total;
       if (pobtas_algo(b, a, rr, 0) == SOLVE_BLOCK || pobtas_algo(b, a, rr, 1) == SOLVE_BLOCK)
         need = std::max(need, bs_layout(n, b, a, rr).total);
+      if (pobtas_algo(b, a, rr, 0) == SOLVE_WIDE) need = std::max(need, wd_layout(n, b, a).total);
       return need;
     }
     case ROCBTA_OP_LOGDET:
@@ -1550,6 +1686,10 @@ int rocbta_dpobtas(int64_t n, int64_t b, int64_t a, const double* D, const doubl
     if (!workspace || workspace_bytes < bs_layout(n, b, a, r).total) return ROCBTA_ERR_WORKSPACE;
     return pobtas_block(n, b, a, D, L, R, T, B, r, ldb, trans == 'N' ? 0 : 1, partial,
                         (char*)workspace, s);
+  }
+  if (algo == SOLVE_WIDE) {
+    if (!workspace || workspace_bytes < wd_layout(n, b, a).total) return ROCBTA_ERR_WORKSPACE;
+    return pobtas_wide(n, b, a, D, L, R, T, B, r, ldb, trans == 'N' ? 0 : 1, (char*)workspace, s);
   }
   if (algo == SOLVE_SWEEP) {
     if (!workspace || workspace_bytes < sw_layout(n, b, a, r).total) return ROCBTA_ERR_WORKSPACE;
