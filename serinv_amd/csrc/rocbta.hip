@@ -974,10 +974,10 @@ int pobtas_block(int64_t n, int64_t b, int64_t a, const double* D, const double*
 }
 
 // Workspace of the wide sweep: the status word first (read by the caller),
-// the block inverses and transposes as for the block sweep, the published
-// rows of one 64-column launch, and the tip's per-block products.
+// the block inverses as for the block sweep, the published rows of one
+// 64-column launch, and the tip's per-block products.
 struct WdLayout {
-  size_t status, linv, lt, w, tinv, wt, sbuf, tbuf, tipp, tipb, total;
+  size_t status, linv, w, tinv, wt, sbuf, tbuf, tipp, tipb, total;
 };
 
 WdLayout wd_layout(int64_t n, int64_t b, int64_t a) {
@@ -991,7 +991,6 @@ WdLayout wd_layout(int64_t n, int64_t b, int64_t a) {
   const int64_t ntb = (b + TT - 1) / TT, nta = (a + TT - 1) / TT;
   l.status = take(sizeof(int));
   l.linv = take((size_t)n * b * b * sizeof(double));
-  l.lt = take((size_t)std::max<int64_t>(n - 1, 1) * b * b * sizeof(double));
   l.w = take((size_t)n * std::max<int64_t>(ntb * TT * TT, ntb * TT * ntb * TT / 4) * sizeof(double));
   l.tinv = take((size_t)a * a * sizeof(double));
   l.wt = take((size_t)nta * TT * TT * sizeof(double));
@@ -1025,23 +1024,8 @@ int pobtas_wide(int64_t n, int64_t b, int64_t a, const double* D, const double* 
     rc = block_trtri(T, a, 0, Tinv, a, a * a, (int)a, 1, (double*)(ws + lay.wt), s);
     if (rc) return rc;
   }
-  if (trans) {  // the backward sweep reads L_{i+1,i}^T and Linv_i^T by rows
-    const int Tt = (int)((b + 63) / 64);
-    for (int64_t z0 = 0; z0 < n; z0 += 65535) {
-      const int nz = (int)std::min<int64_t>(65535, n - z0);
-      transpose_blocks_inplace_kernel<<<dim3(Tt * (Tt + 1) / 2, nz), 256, 0, s>>>(
-          Linv + z0 * b * b, (int)b, b * b);
-      RB_CHECK(hipGetLastError());
-    }
-    for (int64_t z0 = 0; z0 + 1 < n; z0 += 65535) {
-      const int nz = (int)std::min<int64_t>(65535, n - 1 - z0);
-      transpose_blocks_kernel<<<dim3(Tt * Tt, nz), 256, 0, s>>>(
-          L + z0 * b * b, (double*)(ws + lay.lt) + z0 * b * b, (int)b, b * b, Tt);
-      RB_CHECK(hipGetLastError());
-    }
-  }
   WideSweepP P{};
-  P.L = trans ? (const double*)(ws + lay.lt) : L;
+  P.L = L;
   P.Linv = Linv;
   P.ldb = ldb;
   P.n = (int)n;

@@ -13,6 +13,8 @@
 // (y'_i = y_i - R_i^T x_t is formed before the backward launch.)  One
 // launch takes up to WD_CW = 64 right-hand-side columns, so the factor is
 // read once per 64 columns instead of once per 8 as in the block sweep.
+// The backward products read L_{i+1,i} and Linv_i by columns, so no
+// transposed copies are needed.
 //
 // Work split: an item is a 16 x 16 output tile (16 rows of a block, 16 of
 // the columns); workgroup w takes items w, w + G, ... of every phase.  The
@@ -49,8 +51,8 @@ constexpr int WD_MAX_G = 1024;
 constexpr int WD_NB = 8;        // K groups of 16 whose loads one wave issues together
 
 struct WideSweepP {
-  const double* L;     // forward: L, L_{i,i-1} at (i-1) b^2; backward: L_{i+1,i}^T at i b^2
-  const double* Linv;  // forward: Linv_i (zero above); backward: Linv_i^T (zero below)
+  const double* L;     // L_{i+1,i} at i b^2 (backward reads it by columns)
+  const double* Linv;  // Linv_i (zero above; backward reads it by columns)
   double* B;           // right-hand sides, columns [0, r), row stride ldb
   int64_t ldb;
   int n, b, r;         // b % WD_T == 0, r <= WD_CW
@@ -74,16 +76,19 @@ __device__ __forceinline__ bool wd_spin(const WideSweepP& P, long long& spins) {
   return true;
 }
 
-// sum_j M[k0 + row][j] * src[j][c0 + col] over j in [jlo, jhi) (a multiple
-// of 16) for the 16 x 16 tile, into red (the waves' partials) and then
-// out[row * 16 + col] (wave-ordered sum).  M has row stride b; src rows are
-// polled (row stride WD_CW).  Ends with a barrier.
+// sum_j op(M)[k0 + row][j] * src[j][c0 + col] over j in [jlo, jhi) (a
+// multiple of 16) for the 16 x 16 tile, into red (the waves' partials) and
+// then out[row * 16 + col] (wave-ordered sum).  M has row stride b; op(M) =
+// M, or M^T read by columns (TR: for a reduction index, the 16 lanes of a
+// quarter-wave read 128 contiguous bytes of row j); src rows are polled
+// (row stride WD_CW).  Ends with a barrier.
+template <bool TR>
 __device__ __forceinline__ void wd_tile(const WideSweepP& P, const double* M, int k0,
                                         const double* src, int c0, int jlo, int jhi, double* red,
                                         double* out) {
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int li = lane & 15, g4 = (lane >> 4) * 4;
-  const double* mrow = M + (int64_t)(k0 + li) * P.b;
+  const double* mrow = M + (int64_t)(k0 + li) * P.b;  // TR: M^T's row k0 + li is column k0 + li
   f64x4 acc = {0.0, 0.0, 0.0, 0.0};
   // WD_NB groups per round: every load of the round is issued before the
   // first poll, so a round costs one memory round trip
@@ -94,7 +99,8 @@ __device__ __forceinline__ void wd_tile(const WideSweepP& P, const double* M, in
     for (int u = 0; u < WD_NB; ++u) {
       const int j = jr + 16 * WD_NW * u + g4;
 #pragma unroll
-      for (int s = 0; s < 4; ++s) m[u][s] = (j + s < jhi) ? mrow[j + s] : 0.0;
+      for (int s = 0; s < 4; ++s)
+        m[u][s] = (j + s < jhi) ? (TR ? M[(int64_t)(j + s) * P.b + k0 + li] : mrow[j + s]) : 0.0;
     }
 #pragma unroll
     for (int u = 0; u < WD_NB; ++u) {
@@ -167,7 +173,7 @@ __global__ void __launch_bounds__(WD_THREADS) pobtas_wide_kernel(WideSweepP P) {
       if (has_prev) {
         const double* M = TRANS ? P.L + (int64_t)i * bb : P.L + (int64_t)(i - 1) * bb;
         const double* src = P.sbuf + (int64_t)(TRANS ? i + 1 : i - 1) * b * WD_CW;
-        wd_tile(P, M, k0, src, c0, 0, b, red, out);
+        wd_tile<TRANS != 0>(P, M, k0, src, c0, 0, b, red, out);
       } else {
         if (t < 256) out[t] = 0.0;
         __syncthreads();
@@ -185,8 +191,8 @@ __global__ void __launch_bounds__(WD_THREADS) pobtas_wide_kernel(WideSweepP P) {
       const int item = wd_item(w, nitems);
       const int k0 = (item / ntc) * WD_T, c0 = (item % ntc) * WD_T;
       const int jlo = TRANS ? k0 : 0, jhi = TRANS ? b : k0 + WD_T;
-      wd_tile(P, P.Linv + (int64_t)i * bb, k0, P.tbuf + (int64_t)i * b * WD_CW, c0, jlo, jhi, red,
-              out);
+      wd_tile<TRANS != 0>(P, P.Linv + (int64_t)i * bb, k0, P.tbuf + (int64_t)i * b * WD_CW, c0, jlo,
+                          jhi, red, out);
       if (t < 256) {
         const int c = c0 + col;
         const int64_t grow = (int64_t)i * b + k0 + row;
