@@ -1,0 +1,53 @@
+"""rocbta_dpobtas called through the C-ABI on a column range of a wider
+right-hand-side array (row stride ldb > r, pointer at a column offset), for
+each solve algorithm: the block sweep, its wide form, the tile sweep and
+the launch-per-tile form (ROCBTA_POBTAS_ALGO).  Checked against the
+oracle's pobtas (src/serinv/algs/pobtas.py:65-161) on those columns at
+1e-12; the columns outside the range must come back untouched.
+"""
+
+import ctypes
+
+import numpy as np
+import pytest
+
+from oracle import serinv_np
+from serinv_amd.utils.matrices import bta_spd
+from tests._helpers import relerr, to_dev, to_host
+
+pytestmark = pytest.mark.gpu
+TOL = 1e-12
+
+
+@pytest.mark.parametrize("algo,n,b,a,r", [("block", 3, 256, 8, 3), ("wide", 4, 64, 5, 70),
+                                          ("sweep", 3, 70, 4, 9), ("loop", 2, 40, 3, 5)])
+def test_pobtas_strided_columns(gpu_device, monkeypatch, algo, n, b, a, r):
+    import torch
+
+    from serinv_amd import _lib
+    from serinv_amd.algs._common import ptr, stream_of
+
+    monkeypatch.setenv("ROCBTA_POBTAS_ALGO", algo)
+    print(f"strided pobtas {algo} n={n} b={b} a={a} r={r}", flush=True)
+    lib = _lib.load()
+    D, L, R, T, _ = bta_spd(n, b, a, seed=1400 + b + r, n_rhs=1)
+    serinv_np.pobtaf(D, L, R, T)
+    ldb, c0 = r + 23, 11
+    Bfull = np.random.default_rng(r).uniform(-1, 1, (n * b + a, ldb))
+    d, l, rr, t, bf = to_dev(gpu_device, D, L, R, T, Bfull)
+    nbytes = lib.rocbta_workspace_bytes(_lib.OP_POBTAS, n, b, a, r)
+    ws = torch.empty(max(nbytes, 8), dtype=torch.uint8, device=gpu_device)
+    ref = Bfull.copy()
+    sub = ref[:, c0:c0 + r].copy()
+    for trans in (b"N", b"C"):
+        serinv_np.pobtas(D, L, R, T, sub, trans=trans.decode())
+        rc = lib.rocbta_dpobtas(n, b, a, ptr(d), ptr(l), ptr(rr), ptr(t),
+                                bf.data_ptr() + c0 * 8, r, ldb, trans, 0, ws.data_ptr(), nbytes,
+                                stream_of(gpu_device))
+        assert rc == 0
+    torch.cuda.synchronize()
+    assert int(ws[:4].view(torch.int32).item()) == 0  # no scheduling time-out
+    (got,) = to_host(bf)
+    assert relerr(got[:, c0:c0 + r], sub) < TOL
+    assert np.array_equal(got[:, :c0], Bfull[:, :c0])
+    assert np.array_equal(got[:, c0 + r:], Bfull[:, c0 + r:])
