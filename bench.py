@@ -591,7 +591,8 @@ def main():
     args = parse()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU only: the BTA factorization path here is single-GPU "
-              "(see DESIGN.md for how it would shard); run with --gpus 1.", file=sys.stderr)
+              "(the distributed form over several GPUs is not built); run with --gpus 1.",
+              file=sys.stderr)
         sys.exit(2)
 
     import numpy as np
