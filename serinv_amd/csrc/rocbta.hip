@@ -339,14 +339,21 @@ int factor_block_column(double* Dm, int64_t ld, int m, const std::vector<Panel>&
 // Inverse of `batch` lower-triangular m x m matrices (source stride sL,
 // row stride lds) into X (row stride ldx, stride sX), upper part zeroed.
 // W: scratch of batch * ceil(m/64) * 64 * 64 doubles.
+// zero_upper = false (halving path only): the strictly upper 64-tiles of X
+// are left unwritten.  The halving recursion reads X's blocks only as lower
+// triangular operands, whose products skip those tiles, and the diagonal
+// tiles come zero above their diagonal from the tile kernel; callers that
+// read X only on and below the diagonal tiles save the n b^2 fill.
 int block_trtri(const double* Ls, int64_t lds, int64_t sL, double* X, int64_t ldx, int64_t sX,
-                int m, int batch, double* W, hipStream_t s, bool halving = false) {
+                int m, int batch, double* W, hipStream_t s, bool halving = false,
+                bool zero_upper = true) {
   if (m <= 0 || batch <= 0) return ROCBTA_OK;
   const int nt = cdiv(m, TT);
   const int nfull = m / TT;
   const int tail = m - nfull * TT;
   if (ldx != m || sX != (int64_t)m * m) return ROCBTA_ERR_ARG;  // X is a dense stack
-  int rc = fill(X, (int64_t)batch * m * m, 0.0, s);
+  const bool halves = halving && tail == 0 && (nt & (nt - 1)) == 0;
+  int rc = (zero_upper || !halves) ? fill(X, (int64_t)batch * m * m, 0.0, s) : ROCBTA_OK;
   if (rc) return rc;
   rc = tile_factor(1, const_cast<double*>(Ls), lds, (int64_t)TT * (lds + 1), sL, X, ldx,
                    (int64_t)TT * (ldx + 1), sX, TT, nfull, batch, nullptr, 0, 0, 0, 0, s);
@@ -357,7 +364,7 @@ int block_trtri(const double* Ls, int64_t lds, int64_t sL, double* X, int64_t ld
                      0, 0, 0, s);
     if (rc) return rc;
   }
-  if (halving && tail == 0 && (nt & (nt - 1)) == 0) {
+  if (halves) {
     // Recursive halving, bottom-up: with L = [A 0; C B] (halves of order h),
     //   L^{-1} = [A^{-1} 0; -B^{-1} (C A^{-1}) B^{-1}],
     // so level h forms W = C A^{-1}, then X21 = -B^{-1} W, for all m / 2h
@@ -910,7 +917,10 @@ int pobtas_block(int64_t n, int64_t b, int64_t a, const double* D, const double*
   RB_CHECK(hipGetLastError());
   double* Linv = (double*)(ws + lay.linv);
   double* Tinv = (double*)(ws + lay.tinv);
-  int rc = block_trtri(D, b, b * b, Linv, b, b * b, (int)b, (int)n, (double*)(ws + lay.w), s, true);
+  // Linv is read on and below the diagonal tiles only (the backward sweep
+  // reads it after the in-place transpose, on and above them)
+  int rc = block_trtri(D, b, b * b, Linv, b, b * b, (int)b, (int)n, (double*)(ws + lay.w), s, true,
+                       false);
   if (rc) return rc;
   if (a > 0) {
     rc = block_trtri(T, a, 0, Tinv, a, a * a, (int)a, 1, (double*)(ws + lay.wt), s);
@@ -1018,7 +1028,9 @@ int pobtas_wide(int64_t n, int64_t b, int64_t a, const double* D, const double* 
   double* Tinv = (double*)(ws + lay.tinv);
   double* tipp = (double*)(ws + lay.tipp);
   double* tipb = (double*)(ws + lay.tipb);
-  int rc = block_trtri(D, b, b * b, Linv, b, b * b, (int)b, (int)n, (double*)(ws + lay.w), s, true);
+  // Linv is read on and below the diagonal tiles only
+  int rc = block_trtri(D, b, b * b, Linv, b, b * b, (int)b, (int)n, (double*)(ws + lay.w), s, true,
+                       false);
   if (rc) return rc;
   if (a > 0) {
     rc = block_trtri(T, a, 0, Tinv, a, a * a, (int)a, 1, (double*)(ws + lay.wt), s);

@@ -1,9 +1,10 @@
 """rocbta_dpobtas called through the C-ABI on a column range of a wider
 right-hand-side array (row stride ldb > r, pointer at a column offset), for
 each solve algorithm: the block sweep, its wide form, the tile sweep and
-the launch-per-tile form (ROCBTA_POBTAS_ALGO).  Checked against the
-oracle's pobtas (src/serinv/algs/pobtas.py:65-161) on those columns at
-1e-12; the columns outside the range must come back untouched.
+the launch-per-tile form (ROCBTA_POBTAS_ALGO), with the workspace filled
+with NaN beforehand (the solves read only workspace they wrote).  Checked
+against the oracle's pobtas (src/serinv/algs/pobtas.py:65-161) on those
+columns at 1e-12; the columns outside the range must come back untouched.
 """
 
 import ctypes
@@ -20,6 +21,7 @@ TOL = 1e-12
 
 
 @pytest.mark.parametrize("algo,n,b,a,r", [("block", 3, 256, 8, 3), ("wide", 4, 64, 5, 70),
+                                          ("wide", 3, 256, 9, 65), ("block", 2, 128, 0, 1),
                                           ("sweep", 3, 70, 4, 9), ("loop", 2, 40, 3, 5)])
 def test_pobtas_strided_columns(gpu_device, monkeypatch, algo, n, b, a, r):
     import torch
@@ -36,7 +38,10 @@ def test_pobtas_strided_columns(gpu_device, monkeypatch, algo, n, b, a, r):
     Bfull = np.random.default_rng(r).uniform(-1, 1, (n * b + a, ldb))
     d, l, rr, t, bf = to_dev(gpu_device, D, L, R, T, Bfull)
     nbytes = lib.rocbta_workspace_bytes(_lib.OP_POBTAS, n, b, a, r)
-    ws = torch.empty(max(nbytes, 8), dtype=torch.uint8, device=gpu_device)
+    # workspace poisoned with NaN: a read of anything the solve did not write
+    # first turns the result into NaN
+    ws = torch.empty(max(nbytes, 8) // 8 * 8 + 8, dtype=torch.uint8, device=gpu_device)
+    ws.view(torch.float64).fill_(float("nan"))
     ref = Bfull.copy()
     sub = ref[:, c0:c0 + r].copy()
     for trans in (b"N", b"C"):
