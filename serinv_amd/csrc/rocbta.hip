@@ -339,11 +339,12 @@ int factor_block_column(double* Dm, int64_t ld, int m, const std::vector<Panel>&
 // Inverse of `batch` lower-triangular m x m matrices (source stride sL,
 // row stride lds) into X (row stride ldx, stride sX), upper part zeroed.
 // W: scratch of batch * ceil(m/64) * 64 * 64 doubles.
-// zero_upper = false (halving path only): the strictly upper 64-tiles of X
-// are left unwritten.  The halving recursion reads X's blocks only as lower
-// triangular operands, whose products skip those tiles, and the diagonal
-// tiles come zero above their diagonal from the tile kernel; callers that
-// read X only on and below the diagonal tiles save the n b^2 fill.
+// zero_upper = false (halving path, 64-tile products only): the strictly
+// upper 64-tiles of X are left unwritten.  The halving recursion reads X's
+// blocks only as lower triangular operands, whose products clip their K
+// range at the output tile and so skip those tiles, and the diagonal tiles
+// come zero above their diagonal from the tile kernel; callers that read X
+// only on and below the diagonal tiles save the n b^2 fill.
 int block_trtri(const double* Ls, int64_t lds, int64_t sL, double* X, int64_t ldx, int64_t sX,
                 int m, int batch, double* W, hipStream_t s, bool halving = false,
                 bool zero_upper = true) {
@@ -353,7 +354,10 @@ int block_trtri(const double* Ls, int64_t lds, int64_t sL, double* X, int64_t ld
   const int tail = m - nfull * TT;
   if (ldx != m || sX != (int64_t)m * m) return ROCBTA_ERR_ARG;  // X is a dense stack
   const bool halves = halving && tail == 0 && (nt & (nt - 1)) == 0;
-  int rc = (zero_upper || !halves) ? fill(X, (int64_t)batch * m * m, 0.0, s) : ROCBTA_OK;
+  // 128-tiles clip the K range of a triangular operand at 128, which reaches
+  // into the strictly upper 64-tiles: those must then be zero
+  const bool skip = !zero_upper && halves && !gemm_tile128_enabled();
+  int rc = skip ? ROCBTA_OK : fill(X, (int64_t)batch * m * m, 0.0, s);
   if (rc) return rc;
   rc = tile_factor(1, const_cast<double*>(Ls), lds, (int64_t)TT * (lds + 1), sL, X, ldx,
                    (int64_t)TT * (ldx + 1), sX, TT, nfull, batch, nullptr, 0, 0, 0, 0, s);
@@ -1489,7 +1493,8 @@ int rocbta_dpobtasi(int64_t n, int64_t b, int64_t a, double* D, double* L, doubl
     const int hi = npre - k * SI_CHUNK;             // exclusive
     const int lo = std::max(0, hi - SI_CHUNK);      // inclusive
     const int nblk = hi - lo;
-    int rc = block_trtri(D + lo * bb, b, bb, Linv, b, bb, bi, nblk, W, ps, true);
+    // Linv enters G, Gt and S only as a triangular operand
+    int rc = block_trtri(D + lo * bb, b, bb, Linv, b, bb, bi, nblk, W, ps, true, false);
     if (rc) return rc;
     GemmList pre;
     const int ng = std::min<int>(hi, (int)n - 1) - lo;

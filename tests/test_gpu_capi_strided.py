@@ -56,3 +56,29 @@ def test_pobtas_strided_columns(gpu_device, monkeypatch, algo, n, b, a, r):
     assert relerr(got[:, c0:c0 + r], sub) < TOL
     assert np.array_equal(got[:, :c0], Bfull[:, :c0])
     assert np.array_equal(got[:, c0 + r:], Bfull[:, c0 + r:])
+
+
+@pytest.mark.parametrize("n,b,a", [(4, 256, 8), (3, 128, 0), (3, 70, 5)])
+def test_pobtasi_poisoned_workspace(gpu_device, n, b, a):
+    """rocbta_dpobtasi through the C-ABI with its workspace filled with NaN
+    beforehand, against the oracle's pobtasi (src/serinv/algs/pobtasi.py:
+    92-172) at 1e-12: the selected inversion reads only workspace it wrote."""
+    import torch
+
+    from serinv_amd import _lib
+    from serinv_amd.algs._common import ptr, stream_of
+
+    print(f"poisoned pobtasi n={n} b={b} a={a}", flush=True)
+    lib = _lib.load()
+    D, L, R, T, _ = bta_spd(n, b, a, seed=1500 + b, n_rhs=1)
+    serinv_np.pobtaf(D, L, R, T)
+    d, l, rr, t = to_dev(gpu_device, D, L, R, T)
+    nbytes = lib.rocbta_workspace_bytes(_lib.OP_POBTASI, n, b, a, 0)
+    ws = torch.empty(nbytes // 8 * 8 + 8, dtype=torch.uint8, device=gpu_device)
+    ws.view(torch.float64).fill_(float("nan"))
+    assert lib.rocbta_dpobtasi(n, b, a, ptr(d), ptr(l), ptr(rr), ptr(t), 1, ws.data_ptr(), nbytes,
+                               stream_of(gpu_device)) == 0
+    torch.cuda.synchronize()
+    serinv_np.pobtasi(D, L, R, T)
+    for g, o in zip(to_host(d, l, rr, t), (D, L, R, T)):
+        assert relerr(g, o) < TOL
