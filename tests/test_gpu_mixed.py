@@ -75,6 +75,22 @@ def test_mixed_factor_selinv(gpu_device, algo, n, b, a):
         assert relerr(g, o) < 1e-5, f"selected inverse {nm}: {relerr(g, o):.2e}"
 
 
+@pytest.mark.parametrize("n,b,a", [(4, 64, 16), (3, 130, 33), (2, 256, 8)])
+def test_mixed_factor_open_last_block(gpu_device, algo, n, b, a):
+    """pobtaf_mixed(factorize_last_block=False): blocks 0 .. n-2 as the FP64
+    oracle's factor, the last block, its arrow block and the tip as the
+    oracle's Schur-updated values (pobtaf.py:143-159), at the mixed bar."""
+    from serinv_amd.mixed import pobtaf_mixed
+
+    D, L, R, T, _ = bta_spd(n, b, a, seed=95 + n + b)
+    Do, Lo, Ro, To = (x.copy() for x in (D, L, R, T))
+    serinv_np.pobtaf(Do, Lo, Ro, To, factorize_last_block=False)
+    d, l, r, t = to_dev(gpu_device, D, L, R, T)
+    pobtaf_mixed(d, l, r, t, factorize_last_block=False)
+    for nm, g, o in zip("DLRT", to_host(d, l, r, t), (Do, Lo, Ro, To)):
+        assert relerr(g, o) < 1e-5, f"{nm}: {relerr(g, o):.2e}"
+
+
 @pytest.mark.parametrize("n,b,a,r", [(4, 64, 16, 1), (3, 130, 33, 16), (6, 96, 0, 4)])
 def test_refinement_reaches_fp64(gpu_device, algo, n, b, a, r):
     from serinv_amd.mixed import pobtaf_mixed, pobtas_refine
