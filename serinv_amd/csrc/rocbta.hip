@@ -931,11 +931,12 @@ int pobtas_block(int64_t n, int64_t b, int64_t a, const double* D, const double*
     if (rc) return rc;
   }
   if (trans) {
-    // the backward sweep reads L_{i+1,i}^T and Linv_i^T by rows
+    // the backward sweep reads L_{i+1,i}^T and (the upper triangle of)
+    // Linv_i^T by rows
     const int T = (int)((b + 63) / 64);
     for (int64_t z0 = 0; z0 < n; z0 += 65535) {
       const int nz = (int)std::min<int64_t>(65535, n - z0);
-      transpose_blocks_inplace_kernel<<<dim3(T * (T + 1) / 2, nz), 256, 0, s>>>(
+      transpose_lower_to_upper_kernel<<<dim3(T * (T + 1) / 2, nz), 256, 0, s>>>(
           Linv + z0 * b * b, (int)b, b * b);
       RB_CHECK(hipGetLastError());
     }

@@ -67,12 +67,15 @@ __global__ void __launch_bounds__(256) transpose_blocks_kernel(const double* src
   }
 }
 
-// In place: workgroup (pair p, z) swaps the transposes of tiles (I, J) and
-// (J, I), I <= J, p = J (J + 1) / 2 + I; a diagonal tile is transposed
-// within itself.
-__global__ void __launch_bounds__(256) transpose_blocks_inplace_kernel(double* X, int b,
+// The upper triangle of X^T for a lower-triangular X, in place: workgroup
+// (pair p, z) writes tile (I, J) = tile (J, I)^T for I < J and transposes a
+// diagonal tile within itself; the strictly lower tiles are left as they
+// are (half the traffic of the full swap).  For readers of the upper
+// triangle only (the backward block sweep reads Linv_i^T from the diagonal
+// on); the strictly upper tiles need not hold anything beforehand.
+__global__ void __launch_bounds__(256) transpose_lower_to_upper_kernel(double* X, int b,
                                                                        int64_t sblk) {
-  __shared__ double t1[64][65], t2[64][65];
+  __shared__ double t[64][65];
   int J = (int)((__builtin_sqrtf(8.0f * (float)blockIdx.x + 1.0f) - 1.0f) * 0.5f);
   while ((J + 1) * (J + 2) / 2 <= (int)blockIdx.x) ++J;
   while (J * (J + 1) / 2 > (int)blockIdx.x) --J;
@@ -82,17 +85,15 @@ __global__ void __launch_bounds__(256) transpose_blocks_inplace_kernel(double* X
 #pragma unroll
   for (int q = 0; q < 16; ++q) {
     const int r = r0 + 4 * q;
-    const int ra = 64 * I + r, ca = 64 * J + c, rb = 64 * J + r, cb = 64 * I + c;
-    t1[r][c] = (ra < b && ca < b) ? x[(int64_t)ra * b + ca] : 0.0;
-    t2[r][c] = (rb < b && cb < b) ? x[(int64_t)rb * b + cb] : 0.0;
+    const int rb = 64 * J + r, cb = 64 * I + c;
+    t[r][c] = (rb < b && cb < b) ? x[(int64_t)rb * b + cb] : 0.0;
   }
   __syncthreads();
 #pragma unroll
   for (int q = 0; q < 16; ++q) {
     const int r = r0 + 4 * q;
-    const int ra = 64 * I + r, ca = 64 * J + c, rb = 64 * J + r, cb = 64 * I + c;
-    if (rb < b && cb < b) x[(int64_t)rb * b + cb] = t1[c][r];   // tile (J, I) = tile(I, J)^T
-    if (I != J && ra < b && ca < b) x[(int64_t)ra * b + ca] = t2[c][r];
+    const int ra = 64 * I + r, ca = 64 * J + c;
+    if (ra < b && ca < b) x[(int64_t)ra * b + ca] = t[c][r];  // tile (I, J) = tile (J, I)^T
   }
 }
 
